@@ -1,0 +1,240 @@
+// Circuit runner linked against the COMPILED REFERENCE (intel-qs, built from its own
+// sources by oracle/Makefile into oracle/_ref/).  Test infrastructure only: it is used
+// to write golden fixtures in this container and to time the reference's OpenMP path
+// as the CPU baseline.  It is never part of the product.
+//
+// It reads the text circuit format written by iqs_amd/circuits.py, drives
+// iqs::QubitRegister<ComplexDP|ComplexSP> (include/qureg.hpp:127-417) op by op, and writes
+//   * the final state as raw complex amplitudes in DATA order (--state FILE),
+//   * one line "R <op> <values...>" per read-out op (--results FILE, default stdout),
+//   * with --time, "T <op-index> <op> <seconds>" per op to stderr and a total.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "qureg.hpp"
+#include "qaoa_features.hpp"
+
+namespace {
+
+struct Graph {
+  int nv = 0;
+  std::vector<int> u, v;
+  std::vector<double> w;
+};
+
+template <class Type>
+struct Runner {
+  using BT = typename iqs::QubitRegister<Type>::BaseType;
+  iqs::QubitRegister<Type>* psi = nullptr;
+  iqs::QubitRegister<Type>* diag = nullptr;  // QAOA cost diagonal (reference layout)
+  Graph g;
+  FILE* res = stdout;
+  bool timing = false;
+
+  static Type C(double re, double im) { return Type(BT(re), BT(im)); }
+
+  void read_m2(std::istringstream& is, TM2x2<Type>& m) {
+    double a[8];
+    for (int i = 0; i < 8; ++i) is >> a[i];
+    m(0, 0) = C(a[0], a[1]); m(0, 1) = C(a[2], a[3]);
+    m(1, 0) = C(a[4], a[5]); m(1, 1) = C(a[6], a[7]);
+  }
+
+  void ensure_diag() {
+    if (diag) return;
+    int n = (int)psi->NumQubits();
+    diag = new iqs::QubitRegister<Type>(n, "base", 0);
+    bool weighted = false;
+    for (double w : g.w) weighted |= (w != 1.0);
+    if (!weighted) {
+      std::vector<int> adj(n * n, 0);
+      for (size_t e = 0; e < g.u.size(); ++e) adj[g.u[e] * n + g.v[e]] = adj[g.v[e] * n + g.u[e]] = 1;
+      iqs::qaoa::InitializeVectorAsMaxCutCostFunction(*diag, adj);
+    } else {
+      std::vector<BT> adj(n * n, 0);
+      for (size_t e = 0; e < g.u.size(); ++e)
+        adj[g.u[e] * n + g.v[e]] = adj[g.v[e] * n + g.u[e]] = BT(g.w[e]);
+      iqs::qaoa::InitializeVectorAsWeightedMaxCutCostFunction(*diag, adj);
+    }
+  }
+
+  void op(const std::string& name, std::istringstream& is) {
+    unsigned a, b, c;
+    double t, p;
+    TM2x2<Type> m;
+    if (name == "Apply1QubitGate") { is >> a; read_m2(is, m); psi->Apply1QubitGate(a, m); }
+    else if (name == "ApplyRotationX") { is >> a >> t; psi->ApplyRotationX(a, BT(t)); }
+    else if (name == "ApplyRotationY") { is >> a >> t; psi->ApplyRotationY(a, BT(t)); }
+    else if (name == "ApplyRotationZ") { is >> a >> t; psi->ApplyRotationZ(a, BT(t)); }
+    else if (name == "ApplyRotationXY") { is >> a >> p >> t; psi->ApplyRotationXY(a, BT(p), BT(t)); }
+    else if (name == "ApplyPauliX") { is >> a; psi->ApplyPauliX(a); }
+    else if (name == "ApplyPauliY") { is >> a; psi->ApplyPauliY(a); }
+    else if (name == "ApplyPauliZ") { is >> a; psi->ApplyPauliZ(a); }
+    else if (name == "ApplyPauliSqrtX") { is >> a; psi->ApplyPauliSqrtX(a); }
+    else if (name == "ApplyPauliSqrtY") { is >> a; psi->ApplyPauliSqrtY(a); }
+    else if (name == "ApplyPauliSqrtZ") { is >> a; psi->ApplyPauliSqrtZ(a); }
+    else if (name == "ApplyT") { is >> a; psi->ApplyT(a); }
+    else if (name == "ApplyHadamard") { is >> a; psi->ApplyHadamard(a); }
+    else if (name == "ApplyControlled1QubitGate") { is >> a >> b; read_m2(is, m); psi->ApplyControlled1QubitGate(a, b, m); }
+    else if (name == "ApplyCRotationX") { is >> a >> b >> t; psi->ApplyCRotationX(a, b, BT(t)); }
+    else if (name == "ApplyCRotationY") { is >> a >> b >> t; psi->ApplyCRotationY(a, b, BT(t)); }
+    else if (name == "ApplyCRotationZ") { is >> a >> b >> t; psi->ApplyCRotationZ(a, b, BT(t)); }
+    else if (name == "ApplyCPauliX") { is >> a >> b; psi->ApplyCPauliX(a, b); }
+    else if (name == "ApplyCPauliY") { is >> a >> b; psi->ApplyCPauliY(a, b); }
+    else if (name == "ApplyCPauliZ") { is >> a >> b; psi->ApplyCPauliZ(a, b); }
+    else if (name == "ApplyCPauliSqrtZ") { is >> a >> b; psi->ApplyCPauliSqrtZ(a, b); }
+    else if (name == "ApplyCHadamard") { is >> a >> b; psi->ApplyCHadamard(a, b); }
+    else if (name == "ApplyCPhaseRotation") { is >> a >> b >> t; psi->ApplyCPhaseRotation(a, b, BT(t)); }
+    else if (name == "ApplySwap") { is >> a >> b; psi->ApplySwap(a, b); }
+    else if (name == "ApplyISwap") { is >> a >> b; psi->ApplyISwap(a, b); }
+    else if (name == "ApplySqrtISwap") { is >> a >> b; psi->ApplySqrtISwap(a, b); }
+    else if (name == "Apply4thRootISwap") { is >> a >> b; psi->Apply4thRootISwap(a, b); }
+    else if (name == "ApplyISwapRotation") { is >> a >> b; read_m2(is, m); psi->ApplyISwapRotation(a, b, m); }
+    else if (name == "ApplyToffoli") { is >> a >> b >> c; psi->ApplyToffoli(a, b, c); }
+    else if (name == "ApplyDiag") {
+      is >> a >> b;
+      double d[8];
+      for (int i = 0; i < 8; ++i) is >> d[i];
+      TM4x4<Type> m4;
+      for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) m4(i, j) = Type(0, 0);
+      for (int i = 0; i < 4; ++i) m4(i, i) = C(d[2 * i], d[2 * i + 1]);
+      psi->ApplyDiag(a, b, m4);
+    }
+    else if (name == "Apply2QubitGate") {
+      is >> a >> b;
+      TM4x4<Type> m4;
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) { double re, im; is >> re >> im; m4(i, j) = C(re, im); }
+      psi->Apply2QubitGate(a, b, m4);
+    }
+    else if (name == "PermuteQubits") {
+      std::string style; is >> style;
+      std::vector<std::size_t> mp(psi->NumQubits());
+      for (auto& x : mp) is >> x;
+      psi->PermuteQubits(mp, style);
+    }
+    else if (name == "EmulateSwap") { is >> a >> b; psi->EmulateSwap(a, b); }
+    else if (name == "CollapseQubit") { int v; is >> a >> v; psi->CollapseQubit(a, v != 0); }
+    else if (name == "Normalize") { psi->Normalize(); }
+    else if (name == "TurnOnSpecialize") { psi->TurnOnSpecialize(); }
+    else if (name == "TurnOffSpecialize") { psi->TurnOffSpecialize(); }
+    else if (name == "TurnOnSpecializeV2") { psi->TurnOnSpecializeV2(); }
+    else if (name == "TurnOffSpecializeV2") { psi->TurnOffSpecializeV2(); }
+    else if (name == "TurnOnFusion") { is >> a; psi->TurnOnFusion(a); }
+    else if (name == "TurnOffFusion") { psi->TurnOffFusion(); }
+    else if (name == "QaoaPhase") { is >> t; ensure_diag(); iqs::qaoa::ImplementQaoaLayerBasedOnCostFunction(*psi, *diag, BT(t)); }
+    // ---- read-outs ----
+    else if (name == "GetProbability") { is >> a; fprintf(res, "R GetProbability %u %.17g\n", a, (double)psi->GetProbability(a)); }
+    else if (name == "ExpectationValue") {
+      double coeff; unsigned k; is >> coeff >> k;
+      std::vector<unsigned> qs(k), os(k);
+      for (unsigned i = 0; i < k; ++i) is >> qs[i] >> os[i];
+      double v = psi->ExpectationValue(qs, os, BT(coeff));
+      fprintf(res, "R ExpectationValue %.17g\n", v);
+    }
+    else if (name == "ComputeNorm") { fprintf(res, "R ComputeNorm %.17g\n", (double)psi->ComputeNorm()); }
+    else if (name == "GetGlobalAmplitude") {
+      std::size_t k; is >> k;
+      Type v = psi->GetGlobalAmplitude(k);
+      fprintf(res, "R GetGlobalAmplitude %zu %.17g %.17g\n", k, (double)v.real(), (double)v.imag());
+    }
+    else if (name == "QaoaExpectation") {
+      ensure_diag();
+      fprintf(res, "R QaoaExpectation %.17g\n", (double)iqs::qaoa::GetExpectationValueFromCostFunction(*psi, *diag));
+    }
+    else if (name == "ZAll") {
+      fprintf(res, "R ZAll");
+      for (unsigned q = 0; q < psi->NumQubits(); ++q) fprintf(res, " %.17g", (double)psi->ExpectationValueZ(q));
+      fprintf(res, "\n");
+    }
+    else { fprintf(stderr, "ref_driver: unknown op '%s'\n", name.c_str()); std::exit(3); }
+  }
+
+  int run(std::istream& in, int nq, const std::string& init, std::size_t base, const char* state_out) {
+    psi = new iqs::QubitRegister<Type>(nq, "base", 0);
+    if (init == "plus") psi->Initialize("++++", 0);
+    else psi->Initialize("base", base);
+    std::string line;
+    auto t_all0 = std::chrono::steady_clock::now();
+    long idx = 0;
+    while (std::getline(in, line)) {
+      if (line.empty() || line[0] == '#') continue;
+      std::istringstream is(line);
+      std::string name;
+      is >> name;
+      auto t0 = std::chrono::steady_clock::now();
+      op(name, is);
+      if (timing) {
+        double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        fprintf(stderr, "T %ld %s %.9f\n", idx, name.c_str(), dt);
+      }
+      ++idx;
+    }
+    psi->TurnOffFusion();
+    double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_all0).count();
+    if (timing) fprintf(stderr, "TOTAL %ld %.9f\n", idx, total);
+    if (state_out) {
+      FILE* f = std::fopen(state_out, "wb");
+      if (!f) { perror(state_out); return 2; }
+      std::fwrite(psi->RawState(), sizeof(Type), psi->LocalSize(), f);
+      std::fclose(f);
+    }
+    delete psi;
+    delete diag;
+    return 0;
+  }
+};
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  const char* circ = nullptr;
+  const char* state_out = nullptr;
+  const char* res_out = nullptr;
+  bool timing = false;
+  for (int i = 1; i < argc; ++i) {
+    if (!std::strcmp(argv[i], "--state") && i + 1 < argc) state_out = argv[++i];
+    else if (!std::strcmp(argv[i], "--results") && i + 1 < argc) res_out = argv[++i];
+    else if (!std::strcmp(argv[i], "--time")) timing = true;
+    else circ = argv[i];
+  }
+  if (!circ) { fprintf(stderr, "usage: ref_driver CIRCUIT [--state F] [--results F] [--time]\n"); return 1; }
+  std::ifstream in(circ);
+  if (!in) { perror(circ); return 1; }
+  std::string line, tag;
+  int nq = 0;
+  std::string prec = "dp", init = "base";
+  std::size_t base = 0;
+  Graph g;
+  // Header: "iqs-circuit 1", then key lines until "begin".
+  while (std::getline(in, line)) {
+    std::istringstream is(line);
+    is >> tag;
+    if (tag == "begin") break;
+    if (tag == "qubits") is >> nq;
+    else if (tag == "precision") is >> prec;
+    else if (tag == "init") { is >> init; if (init == "base") is >> base; }
+    else if (tag == "graph") {
+      int ne; is >> g.nv >> ne;
+      for (int e = 0; e < ne; ++e) { int a, b; double w; is >> a >> b >> w; g.u.push_back(a); g.v.push_back(b); g.w.push_back(w); }
+    }
+  }
+  FILE* res = res_out ? std::fopen(res_out, "w") : stdout;
+  int rc;
+  if (prec == "sp") {
+    Runner<ComplexSP> r; r.res = res; r.timing = timing; r.g = g;
+    rc = r.run(in, nq, init, base, state_out);
+  } else {
+    Runner<ComplexDP> r; r.res = res; r.timing = timing; r.g = g;
+    rc = r.run(in, nq, init, base, state_out);
+  }
+  if (res != stdout) std::fclose(res);
+  return rc;
+}

@@ -1,0 +1,224 @@
+"""Write the golden fixtures in this folder by running circuits through the COMPILED
+REFERENCE (oracle/_ref/ref_driver, built by `make -C oracle ref` from the reference's own
+sources).  Run in the development container, where the reference sources exist:
+
+    python tests/golden/make_golden.py
+
+Each case writes <name>.circ (the gate list), <name>.res (the reference's read-out lines)
+and <name>.npz with either the full final state ("state") or, for large registers, sampled
+amplitudes ("idx", "amp") plus FP64 checksums ("norm2", "sum").  Reductions are made with
+OMP_NUM_THREADS=1 so that their order is the reference's serial order.
+"""
+from __future__ import annotations
+
+import math
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "intel-qs_amd"))
+
+from iqs_amd.circuits import (Circuit, SplitMix64, brick_circuit, format_expectation,  # noqa: E402
+                              haar_unitary, qaoa_circuit, random_1q2q_circuit,
+                              random_regular_graph, random_su2)
+
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+
+
+def prep(c: Circuit, rng: SplitMix64) -> None:
+    """Generic entangled start state: H and random rotations everywhere, CNOT chain."""
+    n = c.num_qubits
+    for q in range(n):
+        c.add("ApplyHadamard", q)
+        c.add("Apply1QubitGate", q, random_su2(rng))
+    for q in range(n - 1):
+        c.add("ApplyCPauliX", q, q + 1)
+    for q in range(n):
+        c.add("ApplyRotationY", q, rng.uniform(0, 2 * math.pi))
+
+
+def readout(c: Circuit) -> None:
+    c.add("ZAll")
+    c.add("ComputeNorm")
+    c.add("GetGlobalAmplitude", 5)
+
+
+def case_gates1q(prec: str, spec2: bool = False) -> Circuit:
+    rng = SplitMix64(101)
+    c = Circuit(6, precision=prec)
+    if spec2:
+        c.add("TurnOnSpecializeV2")
+    prep(c, rng)
+    for q in range(6):
+        for name in ("ApplyHadamard", "ApplyPauliX", "ApplyPauliY", "ApplyPauliZ", "ApplyPauliSqrtX",
+                     "ApplyPauliSqrtY", "ApplyPauliSqrtZ", "ApplyT"):
+            c.add(name, q)
+        for name in ("ApplyRotationX", "ApplyRotationY", "ApplyRotationZ"):
+            c.add(name, q, rng.uniform(-7, 7))
+        c.add("ApplyRotationXY", q, rng.uniform(0, 6), rng.uniform(0, 6))
+        c.add("Apply1QubitGate", q, haar_unitary(2, rng))
+    readout(c)
+    return c
+
+
+def case_gates2q(prec: str, spec2: bool = False) -> Circuit:
+    rng = SplitMix64(202)
+    n = 6
+    c = Circuit(n, precision=prec)
+    if spec2:
+        c.add("TurnOnSpecializeV2")
+    prep(c, rng)
+    pairs = [(0, 1), (1, 0), (0, 5), (5, 0), (2, 4), (4, 2), (3, 1)]
+    for (a, b) in pairs:
+        for name in ("ApplyCPauliX", "ApplyCPauliY", "ApplyCPauliZ", "ApplyCPauliSqrtZ", "ApplyCHadamard"):
+            c.add(name, a, b)
+        for name in ("ApplyCRotationX", "ApplyCRotationY", "ApplyCRotationZ", "ApplyCPhaseRotation"):
+            c.add(name, a, b, rng.uniform(-7, 7))
+        c.add("ApplyControlled1QubitGate", a, b, haar_unitary(2, rng))
+        for name in ("ApplySwap", "ApplyISwap", "ApplySqrtISwap", "Apply4thRootISwap"):
+            c.add(name, a, b)
+        th = rng.uniform(0, 6)
+        c.add("ApplyISwapRotation", a, b, np.array([[math.cos(th), 1j * math.sin(th)], [1j * math.sin(th), math.cos(th)]]))
+        d = np.exp(1j * np.array([rng.uniform(0, 6) for _ in range(4)]))
+        c.add("ApplyDiag", a, b, d)
+        c.add("Apply2QubitGate", a, b, haar_unitary(4, rng))
+    for (a, b, t) in [(0, 1, 2), (5, 3, 0), (2, 4, 5), (1, 5, 3)]:
+        c.add("ApplyToffoli", a, b, t)
+    readout(c)
+    return c
+
+
+def case_permute(prec: str) -> Circuit:
+    rng = SplitMix64(303)
+    n = 7
+    c = Circuit(n, precision=prec)
+    prep(c, rng)
+    c.add("PermuteQubits", "direct", [1, 0, 2, 6, 4, 5, 3])
+    c.add("ApplyHadamard", 3)
+    c.add("ApplyCPauliX", 6, 0)
+    c.add("Apply2QubitGate", 4, 1, haar_unitary(4, rng))  # position order == qubit order
+    c.add("ApplyDiag", 2, 5, np.exp(1j * np.array([0.1, 0.7, -1.3, 2.2])))
+    c.add("PermuteQubits", "inverse", [3, 1, 4, 0, 6, 2, 5])
+    c.add("EmulateSwap", 0, 6)
+    c.add("ApplyRotationX", 0, 0.4)
+    c.add("ApplySwap", 2, 4)
+    c.add("PermuteQubits", "direct", list(range(n)))
+    for k in (0, 5, 77, 127):
+        c.add("GetGlobalAmplitude", k)
+    readout(c)
+    return c
+
+
+def case_expect(prec: str) -> Circuit:
+    rng = SplitMix64(404)
+    n = 6
+    c = Circuit(n, precision=prec)
+    prep(c, rng)
+    for q in range(n):
+        c.add("GetProbability", q)
+    terms = [
+        (1.0, [(0, 1)]), (1.0, [(2, 2)]), (0.5, [(3, 3)]), (2.0, [(0, 1), (1, 1)]),
+        (1.0, [(0, 2), (5, 2)]), (-1.5, [(1, 3), (4, 3)]), (1.0, [(0, 1), (2, 2), (4, 3)]),
+        (1.0, [(5, 1), (3, 2), (1, 3), (0, 1)]), (3.0, []),
+    ]
+    for coeff, pairs in terms:
+        c.ops.append(format_expectation(coeff, pairs))
+    c.add("CollapseQubit", 2, 1)
+    c.add("ComputeNorm")
+    c.add("Normalize")
+    c.add("GetProbability", 2)
+    c.add("CollapseQubit", 4, 0)
+    c.add("Normalize")
+    readout(c)
+    return c
+
+
+def case_qaoa(prec: str, weighted: bool = False) -> Circuit:
+    rng = SplitMix64(505)
+    if weighted:
+        n = 6
+        g = [(0, 1, 0.5), (1, 2, 1.25), (2, 3, 2.0), (3, 4, 0.75), (4, 5, 1.5), (5, 0, 0.3), (0, 3, 1.1)]
+    else:
+        n = 8
+        g = random_regular_graph(n, 3, 20240604)
+    gam = [rng.uniform(0, math.pi) for _ in range(2)]
+    bet = [rng.uniform(0, math.pi) for _ in range(2)]
+    c = qaoa_circuit(n, g, gam, bet, precision=prec)
+    c.add("ZAll")
+    return c
+
+
+CASES = {
+    "gates1q_dp": lambda: case_gates1q("dp"),
+    "gates1q_sp": lambda: case_gates1q("sp"),
+    "spec2_1q_dp": lambda: case_gates1q("dp", True),
+    "spec2_1q_sp": lambda: case_gates1q("sp", True),
+    "gates2q_dp": lambda: case_gates2q("dp"),
+    "gates2q_sp": lambda: case_gates2q("sp"),
+    "spec2_2q_dp": lambda: case_gates2q("dp", True),
+    "spec2_2q_sp": lambda: case_gates2q("sp", True),
+    "permute_dp": lambda: case_permute("dp"),
+    "permute_sp": lambda: case_permute("sp"),
+    "expect_dp": lambda: case_expect("dp"),
+    "expect_sp": lambda: case_expect("sp"),
+    "qaoa_dp": lambda: case_qaoa("dp"),
+    "qaoa_sp": lambda: case_qaoa("sp"),
+    "qaoaw_dp": lambda: case_qaoa("dp", True),
+    "random1q2q_dp": lambda: random_1q2q_circuit(8, 60, 20240603),
+    "random1q2q_sp": lambda: random_1q2q_circuit(8, 60, 20240603, precision="sp"),
+    "brick10_dp": lambda: brick_circuit(10, 8, 77),
+    "brick10_sp": lambda: brick_circuit(10, 8, 77, precision="sp"),
+    # The reference's CPU-runnable headline case: 20 qubits, depth 20, seed 20240601.
+    "brick20_dp": lambda: brick_circuit(20, 20, 20240601),
+}
+
+FULL_STATE_MAX_QUBITS = 10
+SAMPLE_SEED = 99
+
+
+def sample_indices(n: int, k: int = 4096) -> np.ndarray:
+    rng = SplitMix64(SAMPLE_SEED)
+    return np.array(sorted({rng.randint(1 << n) for _ in range(k)}), dtype=np.int64)
+
+
+def run_reference(text: str):
+    with tempfile.TemporaryDirectory() as d:
+        cf, sf, rf = (os.path.join(d, x) for x in ("c.txt", "s.bin", "r.txt"))
+        with open(cf, "w") as f:
+            f.write(text)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([DRIVER, cf, "--state", sf, "--results", rf], check=True, env=env)
+        prec = "sp" if "precision sp" in text else "dp"
+        st = np.fromfile(sf, dtype=np.complex64 if prec == "sp" else np.complex128)
+        res = open(rf).read()
+    return st, res
+
+
+def main() -> None:
+    if not os.path.exists(DRIVER):
+        sys.exit(f"{DRIVER} missing: run `make -C oracle ref` first")
+    for name, fn in CASES.items():
+        c = fn()
+        text = c.to_text()
+        st, res = run_reference(text)
+        with open(os.path.join(HERE, name + ".circ"), "w") as f:
+            f.write(text)
+        with open(os.path.join(HERE, name + ".res"), "w") as f:
+            f.write(res)
+        if c.num_qubits <= FULL_STATE_MAX_QUBITS:
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), state=st)
+        else:
+            idx = sample_indices(c.num_qubits)
+            a = st.astype(np.complex128)
+            np.savez_compressed(os.path.join(HERE, name + ".npz"), idx=idx, amp=st[idx],
+                                norm2=np.array([np.vdot(a, a).real]), sum=np.array([a.sum()]))
+        print(f"{name}: n={c.num_qubits} ops={len(c.ops)} results={len(res.splitlines())}")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Gate-throughput benchmark of the MI355X state-vector engine.
+
+Workload (one GPU): a 30-qubit ComplexDP register (16 GiB in HBM) running the random brick
+circuit of depth 40 (seed 20240602): per layer, H on every qubit, R{X,Y,Z}(theta) on every
+qubit, CNOT(i, i+1) for i = layer mod 2, +2, ...  One step = one layer.  `value` is gates/s
+over the timed layers with the state resident in HBM.
+
+Also reported:
+  roofline      the dominant kernel (dense 1q update, used by H/RX/RY) timed per launch with
+                HIP events on the library stream inside the timed region; algorithmic bytes
+                per launch = 2 * 2^n * 16 B (read + write of the whole state)
+  sweep         GB/s per gate kind at every target position (same byte model as the
+                reference's Timer bookkeeping, src/highperfkernels.cpp:230-231, 370, 478-479)
+  cpu_baseline  the compiled reference (oracle/_ref/ref_driver, OpenMP) on a bounded sample
+                of the same circuit on this host's cores
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "intel-qs_amd"))
+
+METRIC = "30q gate/s & %HBM peak; QAOA-28q grad s: adjoint vs param-shift @1e-6"
+HBM_PEAK_GBS = 8000.0
+
+
+def apply_op(sv, op, G):
+    name = op[0]
+    if name in G.NAMED_1Q:
+        sv.apply_1q(op[1], G.NAMED_1Q[name]())
+    elif name in G.ROT_1Q:
+        sv.apply_1q(op[1], G.ROT_1Q[name](op[2]))
+    elif name in G.NAMED_C:
+        sv.apply_c1q(op[1], op[2], G.NAMED_C[name]())
+    else:
+        raise ValueError(name)
+
+
+def kernel_kind(op):
+    name = op[0]
+    if name in ("ApplyHadamard", "ApplyRotationX", "ApplyRotationY"):
+        return "dense1q"
+    if name == "ApplyRotationZ":
+        return "diag1q"
+    return "cnot"
+
+
+def cpu_baseline(n: int, layer_ops, budget_s: float = 30.0):
+    """Time the compiled reference on layer 0 of the same circuit (bounded by budget_s)."""
+    drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if not os.path.exists(drv):
+        return None
+    from iqs_amd.circuits import Circuit
+    c = Circuit(n)
+    for op in layer_ops:
+        c.add(*op)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    with tempfile.TemporaryDirectory() as d:
+        cf = os.path.join(d, "c.txt")
+        c.write(cf)
+        try:
+            r = subprocess.run([drv, cf, "--time"], capture_output=True, text=True, timeout=budget_s * 6,
+                               env=dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread",
+                                        OMP_PLACES="cores"))
+        except subprocess.TimeoutExpired:
+            return None
+    times = [float(l.split()[3]) for l in r.stderr.splitlines() if l.startswith("T ")]
+    if not times:
+        return None
+    return {"value": len(times) / sum(times), "unit": "gates/s", "cores": threads, "kind": "reference",
+            "sample": f"{n}-qubit ComplexDP brick layer 0 ({len(times)} gates) through the compiled reference "
+                      f"(intel-qs src/, -O3 OpenMP), per-gate steady_clock"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--sweep-reps", type=int, default=3)
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: --gpus {args.gpus} is not supported: this engine keeps the whole register in one "
+              f"GPU's HBM and runs on one GPU only (use --gpus 1).", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        return 1
+    from iqs_amd import gates as G
+    from iqs_amd._lib import call
+    from iqs_amd.circuits import SplitMix64, brick_layer_ops, haar_unitary, random_su2
+    from iqs_amd.engine import StateVector
+
+    torch.cuda.init()
+    stream = torch.cuda.current_stream()
+    call("iqsh_set_stream", stream.cuda_stream)
+
+    n = args.qubits
+    S = (1 << n) * 16
+    sv = StateVector(n, "c128")
+
+    rng = SplitMix64(20240602)
+    layers = [brick_layer_ops(n, l, rng) for l in range(args.steps)]
+    wrng = SplitMix64(1)
+    warm = [brick_layer_ops(n, l, wrng) for l in range(args.warmup)]
+
+    sv.init_basis(0)
+    for ops in warm:
+        for op in ops:
+            apply_op(sv, op, G)
+    sv.init_basis(0)
+    torch.cuda.synchronize()
+
+    # ---- timed region: K layers, an event pair around every gate ----
+    ev = []
+    t0 = time.perf_counter()
+    for ops in layers:
+        for op in ops:
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            apply_op(sv, op, G)
+            b.record(stream)
+            ev.append((kernel_kind(op), a, b))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ngates = len(ev)
+    per_kind = {}
+    for kind, a, b in ev:
+        per_kind.setdefault(kind, []).append(a.elapsed_time(b) * 1e-3)
+    dense = per_kind["dense1q"]
+    dense_avg = sum(dense) / len(dense)
+    achieved = 2 * S / dense_avg / 1e9
+
+    readout = [1.0 - 2.0 * sv.prob1(q) for q in range(n)]
+
+    # ---- per-kind sweep over every position ----
+    sweep = {}
+    if not args.no_sweep:
+        srng = SplitMix64(7)
+        kinds = {
+            "1q_dense": ([(p,) for p in range(n)], lambda p: sv.apply_1q(p[0], random_su2(srng)), 2 * S),
+            "controlled_1q": ([((t + 1) % n, t) for t in range(n)] + [(0, n - 1), (n - 1, 0)],
+                              lambda p: sv.apply_c1q(p[0], p[1], random_su2(srng)), S),
+            "diag_2q": ([(p, (p + 1) % n) for p in range(n)],
+                        lambda p: sv.apply_diag2(p[0], p[1], np.exp(1j * np.array([srng.uniform(0, 6) for _ in range(4)]))), 2 * S),
+            "swap": ([(p, n - 1 - p) for p in range(n // 2)], lambda p: sv.apply_swaplike(p[0], p[1], G.SWAP), S),
+            "dense_2q": ([(p, p + 1) for p in range(n - 1)] + [(0, n - 1)],
+                         lambda p: sv.apply_2q(p[0], p[1], haar_unitary(4, srng)), 2 * S),
+        }
+        for kname, (targets, fn, nbytes) in kinds.items():
+            rows = []
+            for tgt in targets:
+                fn(tgt)  # warm-up
+                a = torch.cuda.Event(enable_timing=True)
+                b = torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                for _ in range(args.sweep_reps):
+                    fn(tgt)
+                b.record(stream)
+                b.synchronize()
+                t = a.elapsed_time(b) * 1e-3 / args.sweep_reps
+                rows.append((tgt, t))
+            ts = [t for _, t in rows]
+            sweep[kname] = {
+                "bytes_per_gate": nbytes,
+                "avg_ms": 1e3 * sum(ts) / len(ts),
+                "min_GBs": nbytes / max(ts) / 1e9,
+                "avg_GBs": nbytes / (sum(ts) / len(ts)) / 1e9,
+                "max_GBs": nbytes / min(ts) / 1e9,
+                "per_target_GBs": [round(nbytes / t / 1e9, 1) for _, t in rows],
+            }
+
+    cpu = None if args.no_cpu else cpu_baseline(n, layers[0])
+
+    out = {
+        "metric": METRIC,
+        "value": ngates / elapsed,
+        "unit": "gates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{n}-qubit ComplexDP random brick circuit, depth {args.steps} (H, R{{X,Y,Z}}, CNOT ladder "
+                               f"per layer), seed 20240602, state resident in HBM",
+                   "qubits": n, "depth": args.steps, "gates": ngates},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_pair<double,1> (dense 1q update: H, RX, RY)",
+                     "avg_launch_ms": 1e3 * dense_avg, "launches": len(dense)},
+        "per_kind_avg_ms": {k: 1e3 * sum(v) / len(v) for k, v in per_kind.items()},
+        "hbm_frac_1q": achieved / HBM_PEAK_GBS,
+        "z_readout_first4": readout[:4],
+        "sweep": sweep,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,181 @@
+/* iqs_hip.h -- C ABI of the MI355X state-vector engine (libiqs_hip.so).
+ *
+ * This is the boundary between host bookkeeping (the iqs::QubitRegister drop-in, the
+ * Python bindings) and the hand-written gfx950 kernels.  Plain pointers and sizes only.
+ *
+ * Conventions
+ *   - A state is 2^n complex amplitudes in HBM, in DATA order: bit k of an index is the
+ *     qubit stored at data position k (the reference's `position`, include/qureg.hpp:83-95).
+ *     Program-qubit -> position maps stay on the host; every entry point takes positions.
+ *   - Complex numbers cross the ABI as interleaved double pairs (re, im); matrices are
+ *     row-major (TinyMatrix layout, include/tinymatrix.hpp:32-35).  For IQSH_C64 states the
+ *     library narrows them to float once per call.
+ *   - 4x4 / 2-qubit index convention: row = 2*b(first position) + b(second position)
+ *     (src/qureg_apply2qubitgate.cpp:53-62, src/qureg_applydiag.cpp:165-171).
+ *   - Every function returns IQSH_OK (0) or an error code, never aborts or throws;
+ *     iqsh_last_error() describes the last failure of the calling thread.
+ *   - Work is stream-ordered on the library stream (iqsh_set_stream).  Functions that
+ *     return host values synchronize that stream; the others are asynchronous.
+ *   - Reductions accumulate in FP64 for both precisions, in a fixed two-stage order
+ *     (bitwise reproducible run to run).
+ */
+#ifndef IQS_HIP_H
+#define IQS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum iqsh_status {
+  IQSH_OK = 0,
+  IQSH_ERR_ARG = 1,    /* invalid argument (position, size, handle, dtype mismatch) */
+  IQSH_ERR_ALLOC = 2,  /* device allocation failed */
+  IQSH_ERR_HIP = 3,    /* HIP runtime error */
+  IQSH_ERR_NOTSUP = 4  /* unsupported combination */
+};
+
+enum iqsh_dtype {
+  IQSH_C64 = 0,  /* std::complex<float>  (ComplexSP, include/utils.hpp:24) */
+  IQSH_C128 = 1  /* std::complex<double> (ComplexDP, include/utils.hpp:25) */
+};
+
+/* SpecializeV2 gate kinds, numbered as iqs::GateSpec1Q / GateSpec2Q (include/gate_spec.hpp:10-25);
+ * IQSH_SPEC_CPHASE is GateSpec2Q::CPhase. */
+enum iqsh_spec {
+  IQSH_SPEC_H = 0, IQSH_SPEC_RX, IQSH_SPEC_RY, IQSH_SPEC_RZ,
+  IQSH_SPEC_X, IQSH_SPEC_Y, IQSH_SPEC_Z, IQSH_SPEC_T_OR_CPHASE
+};
+
+typedef struct iqsh_state *iqsh_handle;
+
+/* ---- errors, device, stream ------------------------------------------------------- */
+const char *iqsh_last_error(void);
+/* Use `stream` (a hipStream_t, NULL = library default stream) for all later work. */
+int iqsh_set_stream(void *stream);
+int iqsh_sync(void);
+/* Compute units of the current device (0 if unknown). */
+int iqsh_device_cus(int *cus);
+/* Timing on the library stream: record a start event; then record a stop event, wait for
+ * it and return the elapsed milliseconds (hipEventElapsedTime). */
+int iqsh_timer_start(void);
+int iqsh_timer_stop(double *ms);
+
+/* ---- lifetime and data movement -------------------------------------------------- */
+/* Allocate a register of n qubits.  Replaces the reference's _mm_malloc allocation
+ * (src/qureg_init.cpp:149-188).  Contents are undefined until initialized. */
+int iqsh_create(unsigned num_qubits, int dtype, iqsh_handle *out);
+/* Wrap existing device memory of 2^n amplitudes (not owned, not freed by destroy).
+ * Counterpart of the importing constructor QubitRegister(n, Type*) (src/qureg_init.cpp:193-200). */
+int iqsh_wrap(unsigned num_qubits, int dtype, void *device_ptr, iqsh_handle *out);
+int iqsh_destroy(iqsh_handle h);
+int iqsh_info(iqsh_handle h, unsigned *num_qubits, int *dtype, void **device_ptr);
+/* Host <-> device copies of `count` amplitudes starting at data index `offset`
+ * (operator[], GetGlobalAmplitude/SetGlobalAmplitude, src/qureg_utils.cpp:78-121). */
+int iqsh_upload(iqsh_handle h, const void *host, uint64_t offset, uint64_t count);
+int iqsh_download(iqsh_handle h, void *host, uint64_t offset, uint64_t count);
+/* dst := src (copy constructor, src/qureg_init.cpp:364-379). */
+int iqsh_copy(iqsh_handle dst, iqsh_handle src);
+
+/* ---- initialization (Initialize("base"|"++++"), src/qureg_init.cpp:225-359) ------- */
+int iqsh_init_basis(iqsh_handle h, uint64_t data_index);
+/* Every amplitude := (re, im) (InitializationWithSameAmplitudeEverywhere, qureg_utils.cpp:173-184). */
+int iqsh_init_uniform(iqsh_handle h, double re, double im);
+
+/* ---- gates -------------------------------------------------------------------------- */
+/* 2x2 m on pairs (i, i + 2^pos), bit pos of i clear: Loop_DN (src/highperfkernels.cpp:287-380)
+ * as called by Apply1QubitGate_helper (src/qureg_apply1qubitgate.cpp:173-230).  Matrix
+ * structure is exploited exactly (identity: no pass; diag with m00 = 1: half the state). */
+int iqsh_apply_1q(iqsh_handle h, unsigned pos, const double m[8]);
+/* SpecializeV2 1q bodies (src/spec_kernels.cpp:36-102, Loop_DN spec :249-332). */
+int iqsh_apply_1q_spec(iqsh_handle h, unsigned pos, int spec, double angle);
+/* Controlled 2x2 on target tpos where bit cpos = 1: Loop_TN (src/highperfkernels.cpp:397-486)
+ * as called by ApplyControlled1QubitGate_helper (src/qureg_applyctrl1qubitgate.cpp:228-406). */
+int iqsh_apply_c1q(iqsh_handle h, unsigned cpos, unsigned tpos, const double m[8]);
+/* SpecializeV2 controlled bodies (src/spec_kernels.cpp:104-153, Loop_TN spec :334-417). */
+int iqsh_apply_c1q_spec(iqsh_handle h, unsigned cpos, unsigned tpos, int spec, double angle);
+/* 2x2 m on a target with several control positions (ctrl_mask bits all 1).  The reference
+ * reaches this through ApplyToffoli's five controlled gates (src/qureg_applytoffoli.cpp:24-47). */
+int iqsh_apply_mc1q(iqsh_handle h, uint64_t ctrl_mask, unsigned tpos, const double m[8]);
+/* m on (|bit p1 = 1, bit p2 = 0>, |bit p1 = 0, bit p2 = 1>), p1 < p2 after ordering:
+ * ApplySwap_helper (src/qureg_applyswap.cpp:114-234). */
+int iqsh_apply_swaplike(iqsh_handle h, unsigned p1, unsigned p2, const double m[8]);
+/* Diagonal d[2*b(p1) + b(p2)] (4 complex): ApplyDiag (src/qureg_applydiag.cpp:55-227). */
+int iqsh_apply_diag2(iqsh_handle h, unsigned p1, unsigned p2, const double d[8]);
+/* Dense 4x4 with row index 2*b(ph) + b(pl): Apply2QubitGate (src/qureg_apply2qubitgate.cpp:15-73). */
+int iqsh_apply_2q(iqsh_handle h, unsigned ph, unsigned pl, const double m[32]);
+/* Dense 2^k x 2^k block on positions pos[0..k-1] (row-major, row index bit j = bit pos[j]),
+ * 1 <= k <= 5.  Replaces the reference's fused-gate replay (src/qureg_fusion.cpp:56-94):
+ * a run of gates is multiplied into one block on the host and applied in one pass. */
+int iqsh_apply_kq(iqsh_handle h, unsigned k, const unsigned *pos, const double *U);
+/* state *= f (AmplitudeWiseScalarMultiplication / Normalize, src/qureg_utils.cpp:162-201). */
+int iqsh_scale(iqsh_handle h, double re, double im);
+/* h += f * x (AmplitudeWiseSum, src/qureg_utils.cpp:208-231). */
+int iqsh_axpy(iqsh_handle h, iqsh_handle x, double re, double im);
+/* Zero the half with bit pos != value (CollapseQubit, src/qureg_measure.cpp:93-126). */
+int iqsh_collapse(iqsh_handle h, unsigned pos, int value);
+
+/* ---- permutation (PermuteLocalQubits, src/qureg_permute.cpp:57-105) ------------------ */
+/* New state: bit b of each new data index is taken from bit src_pos[b] of the old index,
+ * i.e. new[j] = old[sum_b bit_b(j) << src_pos[b]].  Bit-exact move.  Uses a second buffer
+ * of the state's size (allocated on first use, kept until destroy). */
+int iqsh_permute(iqsh_handle h, const unsigned *src_pos);
+
+/* ---- reductions (FP64) --------------------------------------------------------------- */
+/* sum |psi_i|^2 over bit pos = 1 (GetProbability, src/qureg_measure.cpp:136-178). */
+int iqsh_prob1(iqsh_handle h, unsigned pos, double *out);
+/* sum |psi_i|^2 (ComputeNorm squared, src/qureg_utils.cpp:236-256). */
+int iqsh_norm2(iqsh_handle h, double *out);
+/* <psi|P|psi> for the Pauli string P with X-or-Y positions xmask, Y-or-Z positions zmask
+ * and ny Y factors, in one read-only pass; *norm2 receives sum |psi|^2 from the same pass.
+ * Replaces ExpectationValue's basis change + parity reduction + undo
+ * (src/qureg_expectval.cpp:102-213). */
+int iqsh_expval_pauli(iqsh_handle h, uint64_t xmask, uint64_t zmask, unsigned ny,
+                      double *value, double *norm2);
+/* <x|h> = sum conj(x_i) h_i (ComputeOverlap, src/qureg_utils.cpp:267-300). */
+int iqsh_overlap(iqsh_handle h, iqsh_handle x, double *re, double *im);
+/* max |h_i - s x_i| (MaxAbsDiff, src/qureg_utils.cpp:40-68); s = (sre, sim). */
+int iqsh_maxabsdiff(iqsh_handle h, iqsh_handle x, double sre, double sim, double *out);
+/* sum |h_i - x_i|^2 (MaxL2NormDiff, src/qureg_utils.cpp:129-157). */
+int iqsh_l2diff2(iqsh_handle h, iqsh_handle x, double *out);
+
+/* ---- QAOA MaxCut (src/qaoa_features.cpp) -------------------------------------------- */
+/* The cut value of a data index x is sum_e w_e [bit u_e(x) != bit v_e(x)], with u_e, v_e
+ * DATA positions (the caller maps program vertices through its permutation).  The cut is
+ * evaluated on the fly from edge-parity tables; no cost register is read. */
+/* psi_i *= exp(-i gamma cut_i) (ImplementQaoaLayerBasedOnCostFunction, :248-264). */
+int iqsh_qaoa_phase(iqsh_handle h, unsigned ne, const unsigned *u, const unsigned *v,
+                    const double *w, double gamma);
+/* sum cut_i |psi_i|^2 (GetExpectationValueFromCostFunction, :273-295). */
+int iqsh_qaoa_expect(iqsh_handle h, unsigned ne, const unsigned *u, const unsigned *v,
+                     const double *w, double *out);
+/* Write cut_i into the real part of a cost register (InitializeVectorAs[Weighted]MaxCut-
+ * CostFunction, :73-239); returns the maximum cut in *max_cut. */
+int iqsh_maxcut_diag(iqsh_handle diag, unsigned ne, const unsigned *u, const unsigned *v,
+                     const double *w, double *max_cut);
+/* Reference form with a cost register: psi_i *= (cos(g d_i), -sin(g d_i)), d_i = Re diag_i. */
+int iqsh_qaoa_phase_diag(iqsh_handle psi, iqsh_handle diag, double gamma);
+/* sum Re(diag_i) |psi_i|^2 (and with squared cost: GetExpectationValueSquaredFromCostFunction). */
+int iqsh_qaoa_expect_diag(iqsh_handle psi, iqsh_handle diag, int squared, double *out);
+
+/* ---- adjoint differentiation (no counterpart in the reference) ---------------------- */
+/* lam_i := cut_i psi_i and *cost := sum cut_i |psi_i|^2 in one pass (lam = H_C psi). */
+int iqsh_qaoa_cost_apply(iqsh_handle psi, iqsh_handle lam, unsigned ne, const unsigned *u,
+                         const unsigned *v, const double *w, double *cost);
+/* Backward step through psi <- exp(-i gamma C) psi for both psi and lam:
+ * *grad := 2 Im <lam| C |psi>, then psi, lam *= exp(+i gamma cut). */
+int iqsh_qaoa_phase_adjoint(iqsh_handle psi, iqsh_handle lam, unsigned ne, const unsigned *u,
+                            const unsigned *v, const double *w, double gamma, double *grad);
+/* Backward step through a 1q gate U on pos for both states: *grad := 2 Im <lam| G |psi>
+ * (G a 2x2 generator, e.g. X/2 for RX), then psi := Uinv psi, lam := Uinv lam.
+ * G == NULL skips the reduction (non-parameterized gate). */
+int iqsh_adjoint_1q(iqsh_handle psi, iqsh_handle lam, unsigned pos, const double Uinv[8],
+                    const double *G, double *grad);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IQS_HIP_H */

@@ -1,0 +1,1380 @@
+// C-ABI implementation of libiqs_hip.so: state lifetime, launch dispatch, reductions,
+// QAOA and adjoint kernels.  See include/iqs_hip.h for the contract of each entry point.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/iqs_hip.h"
+#include "iqs_kernels.hpp"
+
+using namespace iqsk;
+
+struct iqsh_state {
+  unsigned n = 0;
+  int dtype = IQSH_C128;
+  void* d = nullptr;        // amplitudes
+  bool owned = true;
+  void* spare = nullptr;    // second buffer for out-of-place permutation
+  uint64_t size() const { return uint64_t(1) << n; }
+  size_t esz() const { return dtype == IQSH_C128 ? 16 : 8; }
+};
+
+namespace {
+
+thread_local char g_err[512] = "";
+hipStream_t g_stream = nullptr;
+
+int fail(int code, const char* fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) return fail(IQSH_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+#define CHECK_LAUNCH() HIP_TRY(hipGetLastError())
+
+// ---- reduction workspace: partial sums [NV][kMaxBlocks] + result [NV], pinned host copy ----
+constexpr int kMaxRedBlocks = 2048;
+constexpr int kMaxRedVals = 4;
+struct RedWork {
+  double* partial = nullptr;
+  double* result = nullptr;
+  double* host = nullptr;
+};
+RedWork g_red;
+std::mutex g_mu;
+
+int red_ws(RedWork** out)
+{
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_red.partial) {
+    HIP_TRY(hipMalloc(&g_red.partial, sizeof(double) * kMaxRedVals * kMaxRedBlocks));
+    HIP_TRY(hipMalloc(&g_red.result, sizeof(double) * kMaxRedVals));
+    HIP_TRY(hipHostMalloc((void**)&g_red.host, sizeof(double) * kMaxRedVals, hipHostMallocDefault));
+  }
+  *out = &g_red;
+  return IQSH_OK;
+}
+
+// ---- device matrix ring for block gates: pinned staging + device slots + events ----
+constexpr int kRing = 32;
+constexpr size_t kSlotBytes = 32 * 32 * 16;
+struct Ring {
+  char* host = nullptr;
+  char* dev = nullptr;
+  hipEvent_t ev[kRing];
+  bool used[kRing] = {};
+  int next = 0;
+};
+Ring g_ring;
+
+int ring_put(const void* src, size_t bytes, void** dptr, int* slot)
+{
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!g_ring.host) {
+    HIP_TRY(hipHostMalloc((void**)&g_ring.host, kSlotBytes * kRing, hipHostMallocDefault));
+    HIP_TRY(hipMalloc((void**)&g_ring.dev, kSlotBytes * kRing));
+    for (int i = 0; i < kRing; ++i) HIP_TRY(hipEventCreateWithFlags(&g_ring.ev[i], hipEventDisableTiming));
+  }
+  const int s = g_ring.next;
+  g_ring.next = (s + 1) % kRing;
+  if (g_ring.used[s]) HIP_TRY(hipEventSynchronize(g_ring.ev[s]));
+  std::memcpy(g_ring.host + s * kSlotBytes, src, bytes);
+  HIP_TRY(hipMemcpyAsync(g_ring.dev + s * kSlotBytes, g_ring.host + s * kSlotBytes, bytes, hipMemcpyHostToDevice,
+                         g_stream));
+  *dptr = g_ring.dev + s * kSlotBytes;
+  *slot = s;
+  return IQSH_OK;
+}
+
+int ring_done(int slot)
+{
+  HIP_TRY(hipEventRecord(g_ring.ev[slot], g_stream));
+  g_ring.used[slot] = true;
+  return IQSH_OK;
+}
+
+inline unsigned grid_for(uint64_t items, int per_block)
+{
+  const uint64_t g = (items + per_block - 1) / per_block;
+  return (unsigned)std::max<uint64_t>(g, 1);
+}
+
+inline unsigned red_grid(uint64_t items)
+{
+  // Fixed function of the problem size (deterministic summation order).
+  const uint64_t per = (uint64_t)kBlock * 8;
+  return (unsigned)std::min<uint64_t>(std::max<uint64_t>((items + per - 1) / per, 1), kMaxRedBlocks);
+}
+
+int check_h(iqsh_handle h)
+{
+  if (!h || !h->d) return fail(IQSH_ERR_ARG, "null or released state handle");
+  return IQSH_OK;
+}
+
+int check_pos(iqsh_handle h, unsigned p)
+{
+  if (p >= h->n) return fail(IQSH_ERR_ARG, "position %u out of range for %u qubits", p, h->n);
+  return IQSH_OK;
+}
+
+template <typename T>
+Cx<T> cx(const double* a)
+{
+  return Cx<T>{(T)a[0], (T)a[1]};
+}
+
+template <typename T>
+Mat2<T> mat2(const double* m)
+{
+  return Mat2<T>{cx<T>(m), cx<T>(m + 2), cx<T>(m + 4), cx<T>(m + 6)};
+}
+
+bool is_c(const double* a, double re, double im) { return a[0] == re && a[1] == im; }
+
+// ---------------------------------------------------------------------------------------
+// Launchers
+
+constexpr int kP = 4;  // items per thread in streaming kernels
+
+template <typename T, int K>
+int launch_pair(iqsh_handle h, const Ins<K>& ins, uint64_t mask0, uint64_t mask1, const double* m, bool move)
+{
+  using V = typename Vec<T>::t;
+  const uint64_t items = h->size() >> K;
+  const unsigned grid = grid_for(items, kBlock * kP);
+  if (move)
+    k_pair<T, K, kP, true><<<grid, kBlock, 0, g_stream>>>((V*)h->d, items, ins, mask0, mask1, mat2<T>(m));
+  else
+    k_pair<T, K, kP, false><<<grid, kBlock, 0, g_stream>>>((V*)h->d, items, ins, mask0, mask1, mat2<T>(m));
+  CHECK_LAUNCH();
+  return IQSH_OK;
+}
+
+template <typename T, int K>
+int launch_scale(iqsh_handle h, const Ins<K>& ins, uint64_t mask, const double* f)
+{
+  using V = typename Vec<T>::t;
+  const uint64_t items = h->size() >> K;
+  k_scale<T, K, kP><<<grid_for(items, kBlock * kP), kBlock, 0, g_stream>>>((V*)h->d, items, ins, mask, cx<T>(f));
+  CHECK_LAUNCH();
+  return IQSH_OK;
+}
+
+template <int K>
+Ins<K> sorted_ins(const unsigned* p)
+{
+  Ins<K> ins;
+  for (int i = 0; i < K; ++i) ins.p[i] = p[i];
+  std::sort(ins.p, ins.p + K);
+  return ins;
+}
+
+// Dispatch a controlled / multi-controlled pair update: items carry K inserted zero bits.
+template <typename T>
+int pair_dispatch(iqsh_handle h, const std::vector<unsigned>& ins_pos, uint64_t mask0, uint64_t mask1,
+                  const double* m, bool move)
+{
+  const unsigned* p = ins_pos.data();
+  switch (ins_pos.size()) {
+    case 1: return launch_pair<T, 1>(h, sorted_ins<1>(p), mask0, mask1, m, move);
+    case 2: return launch_pair<T, 2>(h, sorted_ins<2>(p), mask0, mask1, m, move);
+    case 3: return launch_pair<T, 3>(h, sorted_ins<3>(p), mask0, mask1, m, move);
+    case 4: return launch_pair<T, 4>(h, sorted_ins<4>(p), mask0, mask1, m, move);
+    default: return fail(IQSH_ERR_NOTSUP, "at most 3 controls are supported");
+  }
+}
+
+template <typename T>
+int scale_dispatch(iqsh_handle h, const std::vector<unsigned>& ins_pos, uint64_t mask, const double* f)
+{
+  const unsigned* p = ins_pos.data();
+  switch (ins_pos.size()) {
+    case 0: return launch_scale<T, 0>(h, Ins<0>{}, 0, f);
+    case 1: return launch_scale<T, 1>(h, sorted_ins<1>(p), mask, f);
+    case 2: return launch_scale<T, 2>(h, sorted_ins<2>(p), mask, f);
+    case 3: return launch_scale<T, 3>(h, sorted_ins<3>(p), mask, f);
+    case 4: return launch_scale<T, 4>(h, sorted_ins<4>(p), mask, f);
+    default: return fail(IQSH_ERR_NOTSUP, "at most 3 controls are supported");
+  }
+}
+
+// A (possibly controlled) 2x2 update on target t with control mask `ctrl` (bits all 1).
+// Matrix structure is used exactly: identity -> nothing; diag(1, d) -> scale the bit-t = 1
+// amplitudes; diag(a, b) -> two scales; X -> move; otherwise the dense pair kernel.
+template <typename T>
+int controlled_2x2(iqsh_handle h, uint64_t ctrl, unsigned t, const double* m)
+{
+  std::vector<unsigned> ins;
+  for (unsigned b = 0; b < h->n; ++b)
+    if ((ctrl >> b) & 1) ins.push_back(b);
+  ins.push_back(t);
+  const uint64_t tb = uint64_t(1) << t;
+  const bool offzero = is_c(m + 2, 0, 0) && is_c(m + 4, 0, 0);
+  if (offzero) {
+    const bool one0 = is_c(m, 1, 0), one1 = is_c(m + 6, 1, 0);
+    if (one0 && one1) return IQSH_OK;
+    if (!one0 && !one1 && ctrl == 0) {
+      // diag(a, b) on one position: one elementwise pass, d[2*b(t) + b(t)] in {a, b}.
+      using V = typename Vec<T>::t;
+      k_diag2<T, kP><<<grid_for(h->size(), kBlock * kP), kBlock, 0, g_stream>>>(
+          (V*)h->d, h->size(), t, t, cx<T>(m), Cx<T>{0, 0}, Cx<T>{0, 0}, cx<T>(m + 6));
+      CHECK_LAUNCH();
+      return IQSH_OK;
+    }
+    int rc = IQSH_OK;
+    if (!one1) rc = scale_dispatch<T>(h, ins, ctrl | tb, m + 6);
+    if (rc == IQSH_OK && !one0) rc = scale_dispatch<T>(h, ins, ctrl, m);
+    return rc;
+  }
+  const bool xmove = is_c(m, 0, 0) && is_c(m + 6, 0, 0) && is_c(m + 2, 1, 0) && is_c(m + 4, 1, 0);
+  return pair_dispatch<T>(h, ins, ctrl, ctrl | tb, m, xmove);
+}
+
+// ---- SpecializeV2 matrices (src/spec_kernels.cpp:172-178, 264-270, 356-361): the gate-typed
+// bodies compute the same 2x2 action; their constants are evaluated here in the register's
+// precision and applied through the structured 2x2 path.
+template <typename T>
+void spec_matrix(int spec, double angle, double* m)
+{
+  const T theta = (T)angle;
+  const T isqrt2 = (T)(1 / std::sqrt(2.0));
+  const T c2 = std::cos(theta / 2), s2 = std::sin(theta / 2);
+  for (int i = 0; i < 8; ++i) m[i] = 0;
+  switch (spec) {
+    case IQSH_SPEC_H: m[0] = m[2] = m[4] = isqrt2; m[6] = -isqrt2; break;
+    case IQSH_SPEC_RX: m[0] = m[6] = c2; m[3] = m[5] = -s2; break;
+    case IQSH_SPEC_RY: m[0] = m[6] = c2; m[2] = -s2; m[4] = s2; break;
+    case IQSH_SPEC_RZ: m[0] = m[6] = c2; m[1] = -s2; m[7] = s2; break;
+    case IQSH_SPEC_X: m[2] = m[4] = 1; break;
+    case IQSH_SPEC_Y: m[3] = -1; m[5] = 1; break;
+    case IQSH_SPEC_Z: m[0] = 1; m[6] = -1; break;
+    default: break;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Reductions: NV running sums per thread (or a max), block partials, one final block.
+
+template <int NV>
+__device__ __forceinline__ void block_sum_store(double (&v)[NV], double* partial)
+{
+  __shared__ double sh[NV][kBlock / 64];
+#pragma unroll
+  for (int r = 0; r < NV; ++r)
+    for (int off = 32; off > 0; off >>= 1) v[r] += __shfl_xor(v[r], off);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int r = 0; r < NV; ++r) sh[r][wid] = v[r];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int r = 0; r < NV; ++r) {
+      double s = 0;
+      for (int w = 0; w < kBlock / 64; ++w) s += sh[r][w];
+      partial[r * kMaxRedBlocks + blockIdx.x] = s;
+    }
+}
+
+__device__ __forceinline__ void block_max_store(double v, double* partial)
+{
+  __shared__ double sh[kBlock / 64];
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = sh[0];
+    for (int w = 1; w < kBlock / 64; ++w) s = fmax(s, sh[w]);
+    partial[blockIdx.x] = s;
+  }
+}
+
+// Second stage: out[r] = sum_b partial[r][b] (or max), fixed order.
+__global__ __launch_bounds__(kBlock) void k_red_final(const double* __restrict__ partial, unsigned nblocks,
+                                                      unsigned nv, int is_max, double* __restrict__ out)
+{
+  __shared__ double sh[kBlock];
+  for (unsigned r = 0; r < nv; ++r) {
+    double s = is_max ? -1.0 : 0.0;
+    for (unsigned b = threadIdx.x; b < nblocks; b += kBlock) {
+      const double x = partial[r * kMaxRedBlocks + b];
+      s = is_max ? fmax(s, x) : s + x;
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w) sh[threadIdx.x] = is_max ? fmax(sh[threadIdx.x], sh[threadIdx.x + w]) : sh[threadIdx.x] + sh[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[r] = sh[0];
+    __syncthreads();
+  }
+}
+
+template <typename V>
+__device__ __forceinline__ double norm2(const V& a)
+{
+  return (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+}
+
+// sum |psi_i|^2 over items (K inserted bits | mask), K = 0 or 1.
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_red_norm(const typename Vec<T>::t* __restrict__ st, uint64_t items,
+                                                     Ins<K> ins, uint64_t mask, double* partial)
+{
+  double v[1] = {0};
+  for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < items; k += (uint64_t)gridDim.x * kBlock)
+    v[0] += norm2(st[item_base(k, ins) | mask]);
+  block_sum_store<1>(v, partial);
+}
+
+// Pauli string expectation + norm in one pass (see iqsh_expval_pauli).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_red_pauli(const typename Vec<T>::t* __restrict__ st, uint64_t N,
+                                                      uint64_t xmask, uint64_t zmask, unsigned ny, double* partial)
+{
+  double v[2] = {0, 0};
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  if (xmask == 0) {
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += stride) {
+      const double p = norm2(st[i]);
+      v[0] += (__popcll(i & zmask) & 1) ? -p : p;
+      v[1] += p;
+    }
+  } else {
+    const unsigned lsb = __ffsll((long long)xmask) - 1;
+    // <P> = sum_k conj(psi_{k^x}) psi_k i^ny (-1)^popc(k & z).  For the pair (j, j^x) with
+    // a = psi_j, b = psi_{j^x}, t = conj(a) b, s = (-1)^popc(j & z):
+    //   term = i^ny s [conj(t) + (-1)^ny t]
+    //        = (-1)^(ny/2) s 2 Re t            (ny even)
+    //        = (-1)^((ny-1)/2) s 2 Im t        (ny odd)
+    const double sgn_even = ((ny / 2) & 1) ? -1.0 : 1.0;
+    const double sgn_odd = (((ny - 1) / 2) & 1) ? -1.0 : 1.0;
+    for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < N / 2; k += stride) {
+      const uint64_t j = ins0(k, lsb), jp = j ^ xmask;
+      const auto a = st[j], b = st[jp];
+      const double tr = (double)a.x * b.x + (double)a.y * b.y;  // Re conj(a) b
+      const double ti = (double)a.x * b.y - (double)a.y * b.x;  // Im conj(a) b
+      const double s = (__popcll(j & zmask) & 1) ? -1.0 : 1.0;
+      v[0] += (ny & 1) ? sgn_odd * s * 2.0 * ti : sgn_even * s * 2.0 * tr;
+      v[1] += norm2(a) + norm2(b);
+    }
+  }
+  block_sum_store<2>(v, partial);
+}
+
+// <x|h> = sum conj(x) h ; l2: sum |h - x|^2 ; maxabs: max |h - s x|
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_red_overlap(const typename Vec<T>::t* __restrict__ h,
+                                                        const typename Vec<T>::t* __restrict__ x, uint64_t N,
+                                                        double* partial)
+{
+  double v[2] = {0, 0};
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (uint64_t)gridDim.x * kBlock) {
+    const auto a = h[i], b = x[i];
+    v[0] += (double)b.x * a.x + (double)b.y * a.y;
+    v[1] += (double)b.x * a.y - (double)b.y * a.x;
+  }
+  block_sum_store<2>(v, partial);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_red_l2diff(const typename Vec<T>::t* __restrict__ h,
+                                                       const typename Vec<T>::t* __restrict__ x, uint64_t N,
+                                                       double* partial)
+{
+  double v[1] = {0};
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (uint64_t)gridDim.x * kBlock) {
+    const auto a = h[i], b = x[i];
+    const double dr = (double)a.x - b.x, di = (double)a.y - b.y;
+    v[0] += dr * dr + di * di;
+  }
+  block_sum_store<1>(v, partial);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_red_maxabs(const typename Vec<T>::t* __restrict__ h,
+                                                       const typename Vec<T>::t* __restrict__ x, uint64_t N,
+                                                       double sr, double si, double* partial)
+{
+  double m = -1.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (uint64_t)gridDim.x * kBlock) {
+    const auto a = h[i], b = x[i];
+    const double dr = (double)a.x - (sr * b.x - si * b.y), di = (double)a.y - (sr * b.y + si * b.x);
+    m = fmax(m, sqrt(dr * dr + di * di));
+  }
+  block_max_store(m, partial);
+}
+
+// ---- QAOA kernels ----
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qaoa_phase(typename Vec<T>::t* __restrict__ st, uint64_t N, CutSpec cs)
+{
+  __shared__ uint64_t tab[8][256];
+  __shared__ double wt[8][256];
+  __shared__ Cx<T> ph[65];
+  build_tables(tab, cs.inc);
+  if (cs.weighted) build_weight_tables(wt, cs);
+  for (unsigned c = threadIdx.x; c <= cs.ne && c < 65; c += kBlock) ph[c] = Cx<T>{(T)cs.ph_re[c], (T)cs.ph_im[c]};
+  __syncthreads();
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (uint64_t)gridDim.x * kBlock) {
+    int ic;
+    const double c = cut_value(tab, wt, cs, i, &ic);
+    Cx<T> f;
+    if (!cs.weighted) f = ph[ic];
+    else {
+      double s, co;
+      sincos(cs.gamma * c, &s, &co);
+      f = Cx<T>{(T)co, (T)-s};
+    }
+    st[i] = cmul(f, st[i]);
+  }
+}
+
+// mode 0: sum cut |psi|^2 ; mode 1: also lam := cut * psi
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qaoa_cost(const typename Vec<T>::t* __restrict__ st,
+                                                      typename Vec<T>::t* __restrict__ lam, uint64_t N, CutSpec cs,
+                                                      int write_lam, double* partial)
+{
+  __shared__ uint64_t tab[8][256];
+  __shared__ double wt[8][256];
+  build_tables(tab, cs.inc);
+  if (cs.weighted) build_weight_tables(wt, cs);
+  __syncthreads();
+  double v[1] = {0};
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (uint64_t)gridDim.x * kBlock) {
+    int ic;
+    const double c = cut_value(tab, wt, cs, i, &ic);
+    const auto a = st[i];
+    v[0] += c * norm2(a);
+    if (write_lam) {
+      typename Vec<T>::t l;
+      l.x = (T)(c * a.x);
+      l.y = (T)(c * a.y);
+      lam[i] = l;
+    }
+  }
+  block_sum_store<1>(v, partial);
+}
+
+// grad partial: sum Im(conj(lam) c psi); then psi, lam *= exp(+i gamma c)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qaoa_phase_adj(typename Vec<T>::t* __restrict__ st,
+                                                           typename Vec<T>::t* __restrict__ lam, uint64_t N, CutSpec cs,
+                                                           double* partial)
+{
+  __shared__ uint64_t tab[8][256];
+  __shared__ double wt[8][256];
+  __shared__ Cx<T> ph[65];
+  build_tables(tab, cs.inc);
+  if (cs.weighted) build_weight_tables(wt, cs);
+  for (unsigned c = threadIdx.x; c <= cs.ne && c < 65; c += kBlock) ph[c] = Cx<T>{(T)cs.ph_re[c], (T)cs.ph_im[c]};
+  __syncthreads();
+  double v[1] = {0};
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (uint64_t)gridDim.x * kBlock) {
+    int ic;
+    const double c = cut_value(tab, wt, cs, i, &ic);
+    const auto a = st[i], l = lam[i];
+    v[0] += c * ((double)l.x * a.y - (double)l.y * a.x);
+    Cx<T> f;
+    if (!cs.weighted) f = ph[ic];
+    else {
+      double s, co;
+      sincos(cs.gamma * c, &s, &co);
+      f = Cx<T>{(T)co, (T)-s};
+    }
+    st[i] = cmul(f, a);
+    lam[i] = cmul(f, l);
+  }
+  block_sum_store<1>(v, partial);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qaoa_diag(typename Vec<T>::t* __restrict__ st,
+                                                      const typename Vec<T>::t* __restrict__ diag, uint64_t N, T gamma)
+{
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (uint64_t)gridDim.x * kBlock) {
+    const T x = gamma * diag[i].x;
+    T s, c;
+    if (sizeof(T) == 8) {
+      double sd, cd;
+      sincos((double)x, &sd, &cd);
+      s = (T)sd;
+      c = (T)cd;
+    } else {
+      float sf, cf;
+      sincosf((float)x, &sf, &cf);
+      s = (T)sf;
+      c = (T)cf;
+    }
+    st[i] = cmul(Cx<T>{c, -s}, st[i]);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_qaoa_diag_expect(const typename Vec<T>::t* __restrict__ st,
+                                                             const typename Vec<T>::t* __restrict__ diag, uint64_t N,
+                                                             int squared, double* partial)
+{
+  double v[1] = {0};
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (uint64_t)gridDim.x * kBlock) {
+    const double d = diag[i].x;
+    v[0] += (squared ? d * d : d) * norm2(st[i]);
+  }
+  block_sum_store<1>(v, partial);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_maxcut_fill(typename Vec<T>::t* __restrict__ diag, uint64_t N, CutSpec cs,
+                                                        double* partial)
+{
+  __shared__ uint64_t tab[8][256];
+  __shared__ double wt[8][256];
+  build_tables(tab, cs.inc);
+  if (cs.weighted) build_weight_tables(wt, cs);
+  __syncthreads();
+  double m = -1.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (uint64_t)gridDim.x * kBlock) {
+    int ic;
+    const double c = cut_value(tab, wt, cs, i, &ic);
+    typename Vec<T>::t d;
+    d.x = (T)c;
+    d.y = 0;
+    diag[i] = d;
+    m = fmax(m, c);
+  }
+  block_max_store(m, partial);
+}
+
+// ---- adjoint 1q step on two states ----
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_adj_1q(typename Vec<T>::t* __restrict__ st, typename Vec<T>::t* __restrict__ lam,
+                                                   uint64_t items, unsigned pos, Mat2<T> uinv, Mat2<double> g,
+                                                   int with_grad, double* partial)
+{
+  double v[1] = {0};
+  const uint64_t hb = uint64_t(1) << pos;
+  for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < items; k += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t i0 = ins0(k, pos), i1 = i0 | hb;
+    const auto p0 = st[i0], p1 = st[i1], l0 = lam[i0], l1 = lam[i1];
+    if (with_grad) {
+      // (G psi) components in double
+      const double g0r = (g.m00.r * p0.x - g.m00.i * p0.y) + (g.m01.r * p1.x - g.m01.i * p1.y);
+      const double g0i = (g.m00.r * p0.y + g.m00.i * p0.x) + (g.m01.r * p1.y + g.m01.i * p1.x);
+      const double g1r = (g.m10.r * p0.x - g.m10.i * p0.y) + (g.m11.r * p1.x - g.m11.i * p1.y);
+      const double g1i = (g.m10.r * p0.y + g.m10.i * p0.x) + (g.m11.r * p1.y + g.m11.i * p1.x);
+      v[0] += ((double)l0.x * g0i - (double)l0.y * g0r) + ((double)l1.x * g1i - (double)l1.y * g1r);
+    }
+    st[i0] = cmad2(uinv.m00, p0, uinv.m01, p1);
+    st[i1] = cmad2(uinv.m10, p0, uinv.m11, p1);
+    lam[i0] = cmad2(uinv.m00, l0, uinv.m01, l1);
+    lam[i1] = cmad2(uinv.m10, l0, uinv.m11, l1);
+  }
+  block_sum_store<1>(v, partial);
+}
+
+// Finish a reduction: second stage + copy nv doubles to host (synchronizes the stream).
+int red_finish(RedWork* w, unsigned nblocks, unsigned nv, int is_max, double* out)
+{
+  k_red_final<<<1, kBlock, 0, g_stream>>>(w->partial, nblocks, nv, is_max, w->result);
+  CHECK_LAUNCH();
+  HIP_TRY(hipMemcpyAsync(w->host, w->result, sizeof(double) * nv, hipMemcpyDeviceToHost, g_stream));
+  HIP_TRY(hipStreamSynchronize(g_stream));
+  for (unsigned r = 0; r < nv; ++r) out[r] = w->host[r];
+  return IQSH_OK;
+}
+
+int build_cut(iqsh_handle h, unsigned ne, const unsigned* u, const unsigned* v, const double* w, double gamma,
+              CutSpec* cs)
+{
+  if (ne > 64) return fail(IQSH_ERR_NOTSUP, "at most 64 edges are supported (got %u)", ne);
+  std::memset(cs, 0, sizeof *cs);
+  cs->ne = ne;
+  cs->gamma = gamma;
+  cs->inc.nbytes = (h->n + 7) / 8;
+  bool weighted = false;
+  for (unsigned e = 0; e < ne; ++e) {
+    if (u[e] >= h->n || v[e] >= h->n || u[e] == v[e]) return fail(IQSH_ERR_ARG, "bad edge %u: (%u, %u)", e, u[e], v[e]);
+    cs->inc.c[u[e]] ^= uint64_t(1) << e;
+    cs->inc.c[v[e]] ^= uint64_t(1) << e;
+    const double we = w ? w[e] : 1.0;
+    cs->w[e] = we;
+    weighted |= (we != 1.0);
+  }
+  cs->weighted = weighted;
+  // exp(-i gamma c) for integer cuts, evaluated as the reference does per amplitude:
+  // (cos(gamma * c), -sin(gamma * c)) in the register's precision.
+  for (unsigned c = 0; c <= ne; ++c) {
+    if (h->dtype == IQSH_C128) {
+      cs->ph_re[c] = std::cos(gamma * (double)c);
+      cs->ph_im[c] = -std::sin(gamma * (double)c);
+    } else {
+      const float x = (float)gamma * (float)c;
+      cs->ph_re[c] = std::cos(x);
+      cs->ph_im[c] = -std::sin(x);
+    }
+  }
+  return IQSH_OK;
+}
+
+}  // namespace
+
+// =======================================================================================
+// C ABI
+
+#define DISPATCH(h, CALL)                                   \
+  ((h)->dtype == IQSH_C128 ? CALL<double> : CALL<float>)
+
+// Definitions take C linkage from their declarations in iqs_hip.h.
+
+const char* iqsh_last_error(void) { return g_err; }
+
+int iqsh_set_stream(void* stream)
+{
+  g_stream = (hipStream_t)stream;
+  return IQSH_OK;
+}
+
+int iqsh_sync(void)
+{
+  HIP_TRY(hipStreamSynchronize(g_stream));
+  return IQSH_OK;
+}
+
+int iqsh_device_cus(int* cus)
+{
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  HIP_TRY(hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev));
+  return IQSH_OK;
+}
+
+static hipEvent_t g_t0 = nullptr, g_t1 = nullptr;
+
+int iqsh_timer_start(void)
+{
+  if (!g_t0) {
+    HIP_TRY(hipEventCreate(&g_t0));
+    HIP_TRY(hipEventCreate(&g_t1));
+  }
+  HIP_TRY(hipEventRecord(g_t0, g_stream));
+  return IQSH_OK;
+}
+
+int iqsh_timer_stop(double* ms)
+{
+  if (!g_t0 || !ms) return fail(IQSH_ERR_ARG, "timer not started");
+  HIP_TRY(hipEventRecord(g_t1, g_stream));
+  HIP_TRY(hipEventSynchronize(g_t1));
+  float f = 0;
+  HIP_TRY(hipEventElapsedTime(&f, g_t0, g_t1));
+  *ms = f;
+  return IQSH_OK;
+}
+
+int iqsh_create(unsigned num_qubits, int dtype, iqsh_handle* out)
+{
+  if (!out) return fail(IQSH_ERR_ARG, "null output handle");
+  *out = nullptr;
+  if (num_qubits < 1 || num_qubits > 40) return fail(IQSH_ERR_ARG, "num_qubits %u out of range [1, 40]", num_qubits);
+  if (dtype != IQSH_C64 && dtype != IQSH_C128) return fail(IQSH_ERR_ARG, "bad dtype %d", dtype);
+  auto* h = new iqsh_state;
+  h->n = num_qubits;
+  h->dtype = dtype;
+  const size_t bytes = h->size() * h->esz();
+  hipError_t e = hipMalloc(&h->d, bytes);
+  if (e != hipSuccess) {
+    delete h;
+    (void)hipGetLastError();
+    return fail(IQSH_ERR_ALLOC, "hipMalloc(%zu bytes) for %u qubits: %s", bytes, num_qubits, hipGetErrorString(e));
+  }
+  *out = h;
+  return IQSH_OK;
+}
+
+int iqsh_wrap(unsigned num_qubits, int dtype, void* device_ptr, iqsh_handle* out)
+{
+  if (!out || !device_ptr) return fail(IQSH_ERR_ARG, "null pointer");
+  if (num_qubits < 1 || num_qubits > 40) return fail(IQSH_ERR_ARG, "num_qubits %u out of range", num_qubits);
+  if (dtype != IQSH_C64 && dtype != IQSH_C128) return fail(IQSH_ERR_ARG, "bad dtype %d", dtype);
+  auto* h = new iqsh_state;
+  h->n = num_qubits;
+  h->dtype = dtype;
+  h->d = device_ptr;
+  h->owned = false;
+  *out = h;
+  return IQSH_OK;
+}
+
+int iqsh_destroy(iqsh_handle h)
+{
+  if (!h) return IQSH_OK;
+  HIP_TRY(hipStreamSynchronize(g_stream));
+  if (h->owned && h->d) HIP_TRY(hipFree(h->d));
+  if (h->spare) HIP_TRY(hipFree(h->spare));
+  delete h;
+  return IQSH_OK;
+}
+
+int iqsh_info(iqsh_handle h, unsigned* nq, int* dtype, void** ptr)
+{
+  if (int rc = check_h(h)) return rc;
+  if (nq) *nq = h->n;
+  if (dtype) *dtype = h->dtype;
+  if (ptr) *ptr = h->d;
+  return IQSH_OK;
+}
+
+int iqsh_upload(iqsh_handle h, const void* host, uint64_t offset, uint64_t count)
+{
+  if (int rc = check_h(h)) return rc;
+  if (!host || offset > h->size() || count > h->size() - offset) return fail(IQSH_ERR_ARG, "upload range out of bounds");
+  HIP_TRY(hipMemcpyAsync((char*)h->d + offset * h->esz(), host, count * h->esz(), hipMemcpyHostToDevice, g_stream));
+  HIP_TRY(hipStreamSynchronize(g_stream));
+  return IQSH_OK;
+}
+
+int iqsh_download(iqsh_handle h, void* host, uint64_t offset, uint64_t count)
+{
+  if (int rc = check_h(h)) return rc;
+  if (!host || offset > h->size() || count > h->size() - offset) return fail(IQSH_ERR_ARG, "download range out of bounds");
+  HIP_TRY(hipMemcpyAsync(host, (const char*)h->d + offset * h->esz(), count * h->esz(), hipMemcpyDeviceToHost, g_stream));
+  HIP_TRY(hipStreamSynchronize(g_stream));
+  return IQSH_OK;
+}
+
+int iqsh_copy(iqsh_handle dst, iqsh_handle src)
+{
+  if (int rc = check_h(dst)) return rc;
+  if (int rc = check_h(src)) return rc;
+  if (dst->n != src->n || dst->dtype != src->dtype) return fail(IQSH_ERR_ARG, "copy between different registers");
+  if (dst == src) return IQSH_OK;
+  HIP_TRY(hipMemcpyAsync(dst->d, src->d, src->size() * src->esz(), hipMemcpyDeviceToDevice, g_stream));
+  return IQSH_OK;
+}
+
+namespace {
+template <typename T>
+int do_fill(iqsh_handle h, double re, double im)
+{
+  const unsigned grid = (unsigned)std::min<uint64_t>(grid_for(h->size(), kBlock), 65536);
+  k_fill<T><<<grid, kBlock, 0, g_stream>>>((typename Vec<T>::t*)h->d, h->size(), Cx<T>{(T)re, (T)im});
+  CHECK_LAUNCH();
+  return IQSH_OK;
+}
+}  // namespace
+
+int iqsh_init_uniform(iqsh_handle h, double re, double im)
+{
+  if (int rc = check_h(h)) return rc;
+  return DISPATCH(h, do_fill)(h, re, im);
+}
+
+int iqsh_init_basis(iqsh_handle h, uint64_t idx)
+{
+  if (int rc = check_h(h)) return rc;
+  if (idx >= h->size()) return fail(IQSH_ERR_ARG, "basis index out of range");
+  if (int rc = iqsh_init_uniform(h, 0, 0)) return rc;
+  const double one[2] = {1.0, 0.0};
+  const float onef[2] = {1.0f, 0.0f};
+  // Small pageable copy, ordered after the fill on the same stream and waited for.
+  HIP_TRY(hipMemcpyAsync((char*)h->d + idx * h->esz(), h->dtype == IQSH_C128 ? (const void*)one : (const void*)onef,
+                         h->esz(), hipMemcpyHostToDevice, g_stream));
+  HIP_TRY(hipStreamSynchronize(g_stream));
+  return IQSH_OK;
+}
+
+int iqsh_apply_1q(iqsh_handle h, unsigned pos, const double m[8])
+{
+  if (int rc = check_h(h)) return rc;
+  if (int rc = check_pos(h, pos)) return rc;
+  if (!m) return fail(IQSH_ERR_ARG, "null matrix");
+  return DISPATCH(h, controlled_2x2)(h, 0, pos, m);
+}
+
+int iqsh_apply_1q_spec(iqsh_handle h, unsigned pos, int spec, double angle)
+{
+  if (int rc = check_h(h)) return rc;
+  if (int rc = check_pos(h, pos)) return rc;
+  double m[8];
+  if (spec == IQSH_SPEC_T_OR_CPHASE) {
+    const double mt[8] = {1, 0, 0, 0, 0, 0, std::cos(M_PI / 4), std::sin(M_PI / 4)};
+    std::memcpy(m, mt, sizeof m);
+  } else if (spec >= IQSH_SPEC_H && spec <= IQSH_SPEC_Z) {
+    if (h->dtype == IQSH_C128) spec_matrix<double>(spec, angle, m);
+    else spec_matrix<float>(spec, angle, m);
+  } else
+    return fail(IQSH_ERR_ARG, "bad spec %d", spec);
+  return DISPATCH(h, controlled_2x2)(h, 0, pos, m);
+}
+
+int iqsh_apply_c1q(iqsh_handle h, unsigned c, unsigned t, const double m[8])
+{
+  if (int rc = check_h(h)) return rc;
+  if (int rc = check_pos(h, c)) return rc;
+  if (int rc = check_pos(h, t)) return rc;
+  if (c == t) return fail(IQSH_ERR_ARG, "control and target coincide");
+  if (!m) return fail(IQSH_ERR_ARG, "null matrix");
+  return DISPATCH(h, controlled_2x2)(h, uint64_t(1) << c, t, m);
+}
+
+int iqsh_apply_c1q_spec(iqsh_handle h, unsigned c, unsigned t, int spec, double angle)
+{
+  if (int rc = check_h(h)) return rc;
+  if (int rc = check_pos(h, c)) return rc;
+  if (int rc = check_pos(h, t)) return rc;
+  if (c == t) return fail(IQSH_ERR_ARG, "control and target coincide");
+  double m[8];
+  if (spec == IQSH_SPEC_T_OR_CPHASE) {
+    // CPhase: pexp = (cos(theta), sin(theta)) in the register precision.
+    if (h->dtype == IQSH_C128) {
+      const double mp[8] = {1, 0, 0, 0, 0, 0, std::cos(angle), std::sin(angle)};
+      std::memcpy(m, mp, sizeof m);
+    } else {
+      const float th = (float)angle;
+      const double mp[8] = {1, 0, 0, 0, 0, 0, (double)std::cos(th), (double)std::sin(th)};
+      std::memcpy(m, mp, sizeof m);
+    }
+  } else if (spec >= IQSH_SPEC_H && spec <= IQSH_SPEC_Z) {
+    if (h->dtype == IQSH_C128) spec_matrix<double>(spec, angle, m);
+    else spec_matrix<float>(spec, angle, m);
+  } else
+    return fail(IQSH_ERR_ARG, "bad spec %d", spec);
+  return DISPATCH(h, controlled_2x2)(h, uint64_t(1) << c, t, m);
+}
+
+int iqsh_apply_mc1q(iqsh_handle h, uint64_t ctrl, unsigned t, const double m[8])
+{
+  if (int rc = check_h(h)) return rc;
+  if (int rc = check_pos(h, t)) return rc;
+  if (ctrl >> h->n) return fail(IQSH_ERR_ARG, "control mask exceeds register");
+  if ((ctrl >> t) & 1) return fail(IQSH_ERR_ARG, "target is also a control");
+  if (__builtin_popcountll(ctrl) > 3) return fail(IQSH_ERR_NOTSUP, "at most 3 controls are supported");
+  if (!m) return fail(IQSH_ERR_ARG, "null matrix");
+  return DISPATCH(h, controlled_2x2)(h, ctrl, t, m);
+}
+
+namespace {
+template <typename T>
+int do_swaplike(iqsh_handle h, unsigned p1, unsigned p2, const double* m)
+{
+  const unsigned ins[2] = {p1, p2};
+  const bool xmove = is_c(m, 0, 0) && is_c(m + 6, 0, 0) && is_c(m + 2, 1, 0) && is_c(m + 4, 1, 0);
+  if (is_c(m, 1, 0) && is_c(m + 6, 1, 0) && is_c(m + 2, 0, 0) && is_c(m + 4, 0, 0)) return IQSH_OK;
+  return launch_pair<T, 2>(h, sorted_ins<2>(ins), uint64_t(1) << p1, uint64_t(1) << p2, m, xmove);
+}
+}  // namespace
+
+int iqsh_apply_swaplike(iqsh_handle h, unsigned p1, unsigned p2, const double m[8])
+{
+  if (int rc = check_h(h)) return rc;
+  if (int rc = check_pos(h, p1)) return rc;
+  if (int rc = check_pos(h, p2)) return rc;
+  if (p1 == p2) return fail(IQSH_ERR_ARG, "swap positions coincide");
+  if (!m) return fail(IQSH_ERR_ARG, "null matrix");
+  if (p1 > p2) std::swap(p1, p2);
+  return DISPATCH(h, do_swaplike)(h, p1, p2, m);
+}
+
+namespace {
+template <typename T>
+int do_diag2(iqsh_handle h, unsigned p1, unsigned p2, const double* d)
+{
+  // Entries exactly 1 leave their amplitudes untouched: with one non-unit entry only a
+  // quarter of the state is read and written.
+  int nonunit = 0, which = -1;
+  for (int k = 0; k < 4; ++k)
+    if (!is_c(d + 2 * k, 1, 0)) {
+      ++nonunit;
+      which = k;
+    }
+  if (nonunit == 0) return IQSH_OK;
+  if (nonunit == 1) {
+    const uint64_t mask = (((uint64_t)(which >> 1)) << p1) | (((uint64_t)(which & 1)) << p2);
+    const unsigned ins[2] = {p1, p2};
+    return launch_scale<T, 2>(h, sorted_ins<2>(ins), mask, d + 2 * which);
+  }
+  using V = typename Vec<T>::t;
+  k_diag2<T, kP><<<grid_for(h->size(), kBlock * kP), kBlock, 0, g_stream>>>(
+      (V*)h->d, h->size(), p1, p2, cx<T>(d), cx<T>(d + 2), cx<T>(d + 4), cx<T>(d + 6));
+  CHECK_LAUNCH();
+  return IQSH_OK;
+}
+}  // namespace
+
+int iqsh_apply_diag2(iqsh_handle h, unsigned p1, unsigned p2, const double d[8])
+{
+  if (int rc = check_h(h)) return rc;
+  if (int rc = check_pos(h, p1)) return rc;
+  if (int rc = check_pos(h, p2)) return rc;
+  if (p1 == p2) return fail(IQSH_ERR_ARG, "diag positions coincide");
+  if (!d) return fail(IQSH_ERR_ARG, "null diagonal");
+  return DISPATCH(h, do_diag2)(h, p1, p2, d);
+}
+
+namespace {
+template <typename T>
+int do_2q(iqsh_handle h, unsigned ph, unsigned pl, const double* m)
+{
+  using V = typename Vec<T>::t;
+  Mat4<T> M;
+  for (int i = 0; i < 16; ++i) M.m[i] = cx<T>(m + 2 * i);
+  const unsigned ins[2] = {ph, pl};
+  const uint64_t items = h->size() >> 2;
+  k_quad<T, 2><<<grid_for(items, kBlock * 2), kBlock, 0, g_stream>>>((V*)h->d, items, sorted_ins<2>(ins),
+                                                                      uint64_t(1) << pl, uint64_t(1) << ph, M);
+  CHECK_LAUNCH();
+  return IQSH_OK;
+}
+}  // namespace
+
+int iqsh_apply_2q(iqsh_handle h, unsigned ph, unsigned pl, const double m[32])
+{
+  if (int rc = check_h(h)) return rc;
+  if (int rc = check_pos(h, ph)) return rc;
+  if (int rc = check_pos(h, pl)) return rc;
+  if (ph == pl) return fail(IQSH_ERR_ARG, "2q positions coincide");
+  if (!m) return fail(IQSH_ERR_ARG, "null matrix");
+  return DISPATCH(h, do_2q)(h, ph, pl, m);
+}
+
+namespace {
+template <typename T, int KQ>
+int do_block_k(iqsh_handle h, const unsigned* pos, const double* U)
+{
+  using V = typename Vec<T>::t;
+  constexpr int D = 1 << KQ;
+  // Reorder rows/columns so that row bit j corresponds to the j-th smallest position.
+  unsigned order[KQ];
+  for (int j = 0; j < KQ; ++j) order[j] = j;
+  std::sort(order, order + KQ, [&](unsigned a, unsigned b) { return pos[a] < pos[b]; });
+  Ins<KQ> ins;
+  for (int j = 0; j < KQ; ++j) ins.p[j] = pos[order[j]];
+  // caller row index r has bit j <-> pos[j]; sorted index s has bit j <-> ins.p[j] = pos[order[j]]
+  auto to_caller = [&](int s) {
+    int r = 0;
+    for (int j = 0; j < KQ; ++j)
+      if ((s >> j) & 1) r |= 1 << order[j];
+    return r;
+  };
+  std::vector<T> buf(2 * D * D);
+  for (int rs = 0; rs < D; ++rs)
+    for (int cs = 0; cs < D; ++cs) {
+      const int r = to_caller(rs), c = to_caller(cs);
+      buf[2 * (rs * D + cs)] = (T)U[2 * (r * D + c)];
+      buf[2 * (rs * D + cs) + 1] = (T)U[2 * (r * D + c) + 1];
+    }
+  void* dU;
+  int slot;
+  if (int rc = ring_put(buf.data(), buf.size() * sizeof(T), &dU, &slot)) return rc;
+  const uint64_t groups = h->size() >> KQ;
+  constexpr int G = kBlock / D;
+  k_block<T, KQ><<<grid_for(groups, G), kBlock, 0, g_stream>>>((V*)h->d, groups, ins, (const V*)dU);
+  CHECK_LAUNCH();
+  return ring_done(slot);
+}
+
+template <typename T>
+int do_block(iqsh_handle h, unsigned k, const unsigned* pos, const double* U)
+{
+  switch (k) {
+    case 1: return do_block_k<T, 1>(h, pos, U);
+    case 2: return do_block_k<T, 2>(h, pos, U);
+    case 3: return do_block_k<T, 3>(h, pos, U);
+    case 4: return do_block_k<T, 4>(h, pos, U);
+    case 5: return do_block_k<T, 5>(h, pos, U);
+    default: return fail(IQSH_ERR_NOTSUP, "block size %u not supported", k);
+  }
+}
+}  // namespace
+
+int iqsh_apply_kq(iqsh_handle h, unsigned k, const unsigned* pos, const double* U)
+{
+  if (int rc = check_h(h)) return rc;
+  if (!pos || !U) return fail(IQSH_ERR_ARG, "null argument");
+  if (k < 1 || k > 5 || k > h->n) return fail(IQSH_ERR_ARG, "block size %u out of range", k);
+  uint64_t seen = 0;
+  for (unsigned j = 0; j < k; ++j) {
+    if (int rc = check_pos(h, pos[j])) return rc;
+    if ((seen >> pos[j]) & 1) return fail(IQSH_ERR_ARG, "repeated position %u", pos[j]);
+    seen |= uint64_t(1) << pos[j];
+  }
+  return DISPATCH(h, do_block)(h, k, pos, U);
+}
+
+int iqsh_scale(iqsh_handle h, double re, double im)
+{
+  if (int rc = check_h(h)) return rc;
+  if (re == 1.0 && im == 0.0) return IQSH_OK;
+  const double f[2] = {re, im};
+  std::vector<unsigned> none;
+  return DISPATCH(h, scale_dispatch)(h, none, 0, f);
+}
+
+namespace {
+template <typename T>
+int do_axpy(iqsh_handle h, iqsh_handle x, double re, double im)
+{
+  using V = typename Vec<T>::t;
+  const unsigned grid = (unsigned)std::min<uint64_t>(grid_for(h->size(), kBlock), 65536);
+  k_axpy<T><<<grid, kBlock, 0, g_stream>>>((V*)h->d, (const V*)x->d, h->size(), Cx<T>{(T)re, (T)im});
+  CHECK_LAUNCH();
+  return IQSH_OK;
+}
+}  // namespace
+
+int iqsh_axpy(iqsh_handle h, iqsh_handle x, double re, double im)
+{
+  if (int rc = check_h(h)) return rc;
+  if (int rc = check_h(x)) return rc;
+  if (h->n != x->n || h->dtype != x->dtype) return fail(IQSH_ERR_ARG, "axpy between different registers");
+  return DISPATCH(h, do_axpy)(h, x, re, im);
+}
+
+int iqsh_collapse(iqsh_handle h, unsigned pos, int value)
+{
+  if (int rc = check_h(h)) return rc;
+  if (int rc = check_pos(h, pos)) return rc;
+  const double zero[2] = {0, 0};
+  std::vector<unsigned> ins{pos};
+  const uint64_t mask = value ? 0 : (uint64_t(1) << pos);
+  return DISPATCH(h, scale_dispatch)(h, ins, mask, zero);
+}
+
+namespace {
+template <typename T>
+int do_permute(iqsh_handle h, const unsigned* src_pos)
+{
+  using V = typename Vec<T>::t;
+  if (!h->owned) return fail(IQSH_ERR_NOTSUP, "permutation of a wrapped (non-owned) state");
+  if (!h->spare) {
+    hipError_t e = hipMalloc(&h->spare, h->size() * h->esz());
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(IQSH_ERR_ALLOC, "permutation buffer: %s", hipGetErrorString(e));
+    }
+  }
+  BitTab bt;
+  std::memset(&bt, 0, sizeof bt);
+  bt.nbytes = (h->n + 7) / 8;
+  for (unsigned b = 0; b < h->n; ++b) bt.c[b] = uint64_t(1) << src_pos[b];
+  const unsigned grid = (unsigned)std::min<uint64_t>(grid_for(h->size(), kBlock * 8), 65536);
+  k_permute<T><<<grid, kBlock, 0, g_stream>>>((V*)h->spare, (const V*)h->d, h->size(), bt);
+  CHECK_LAUNCH();
+  std::swap(h->d, h->spare);
+  return IQSH_OK;
+}
+}  // namespace
+
+int iqsh_permute(iqsh_handle h, const unsigned* src_pos)
+{
+  if (int rc = check_h(h)) return rc;
+  if (!src_pos) return fail(IQSH_ERR_ARG, "null permutation");
+  uint64_t seen = 0;
+  bool ident = true;
+  for (unsigned b = 0; b < h->n; ++b) {
+    if (int rc = check_pos(h, src_pos[b])) return rc;
+    if ((seen >> src_pos[b]) & 1) return fail(IQSH_ERR_ARG, "not a permutation");
+    seen |= uint64_t(1) << src_pos[b];
+    ident &= (src_pos[b] == b);
+  }
+  if (ident) return IQSH_OK;
+  return DISPATCH(h, do_permute)(h, src_pos);
+}
+
+// ---- reductions ----
+namespace {
+template <typename T>
+int do_norm_sub(iqsh_handle h, int K, unsigned pos, double* out)
+{
+  RedWork* w;
+  if (int rc = red_ws(&w)) return rc;
+  using V = typename Vec<T>::t;
+  if (K == 0) {
+    const unsigned g = red_grid(h->size());
+    k_red_norm<T, 0><<<g, kBlock, 0, g_stream>>>((const V*)h->d, h->size(), Ins<0>{}, 0, w->partial);
+    CHECK_LAUNCH();
+    return red_finish(w, g, 1, 0, out);
+  }
+  Ins<1> ins;
+  ins.p[0] = pos;
+  const unsigned g = red_grid(h->size() / 2);
+  k_red_norm<T, 1><<<g, kBlock, 0, g_stream>>>((const V*)h->d, h->size() / 2, ins, uint64_t(1) << pos, w->partial);
+  CHECK_LAUNCH();
+  return red_finish(w, g, 1, 0, out);
+}
+
+template <typename T>
+int do_pauli(iqsh_handle h, uint64_t x, uint64_t z, unsigned ny, double* out2)
+{
+  RedWork* w;
+  if (int rc = red_ws(&w)) return rc;
+  using V = typename Vec<T>::t;
+  const unsigned g = red_grid(x ? h->size() / 2 : h->size());
+  k_red_pauli<T><<<g, kBlock, 0, g_stream>>>((const V*)h->d, h->size(), x, z, ny, w->partial);
+  CHECK_LAUNCH();
+  return red_finish(w, g, 2, 0, out2);
+}
+
+template <typename T>
+int do_two(iqsh_handle h, iqsh_handle x, int kind, double sr, double si, double* out)
+{
+  RedWork* w;
+  if (int rc = red_ws(&w)) return rc;
+  using V = typename Vec<T>::t;
+  const unsigned g = red_grid(h->size());
+  if (kind == 0) {
+    k_red_overlap<T><<<g, kBlock, 0, g_stream>>>((const V*)h->d, (const V*)x->d, h->size(), w->partial);
+    CHECK_LAUNCH();
+    return red_finish(w, g, 2, 0, out);
+  } else if (kind == 1) {
+    k_red_l2diff<T><<<g, kBlock, 0, g_stream>>>((const V*)h->d, (const V*)x->d, h->size(), w->partial);
+    CHECK_LAUNCH();
+    return red_finish(w, g, 1, 0, out);
+  }
+  k_red_maxabs<T><<<g, kBlock, 0, g_stream>>>((const V*)h->d, (const V*)x->d, h->size(), sr, si, w->partial);
+  CHECK_LAUNCH();
+  return red_finish(w, g, 1, 1, out);
+}
+}  // namespace
+
+int iqsh_prob1(iqsh_handle h, unsigned pos, double* out)
+{
+  if (int rc = check_h(h)) return rc;
+  if (int rc = check_pos(h, pos)) return rc;
+  if (!out) return fail(IQSH_ERR_ARG, "null output");
+  return DISPATCH(h, do_norm_sub)(h, 1, pos, out);
+}
+
+int iqsh_norm2(iqsh_handle h, double* out)
+{
+  if (int rc = check_h(h)) return rc;
+  if (!out) return fail(IQSH_ERR_ARG, "null output");
+  return DISPATCH(h, do_norm_sub)(h, 0, 0, out);
+}
+
+int iqsh_expval_pauli(iqsh_handle h, uint64_t xmask, uint64_t zmask, unsigned ny, double* value, double* nrm2)
+{
+  if (int rc = check_h(h)) return rc;
+  if ((xmask | zmask) >> h->n) return fail(IQSH_ERR_ARG, "Pauli mask exceeds register");
+  if ((unsigned)__builtin_popcountll(xmask & zmask) != ny) return fail(IQSH_ERR_ARG, "ny must equal popcount(xmask & zmask)");
+  double r[2];
+  if (int rc = DISPATCH(h, do_pauli)(h, xmask, zmask, ny, r)) return rc;
+  if (value) *value = r[0];
+  if (nrm2) *nrm2 = r[1];
+  return IQSH_OK;
+}
+
+static int two_check(iqsh_handle h, iqsh_handle x)
+{
+  if (int rc = check_h(h)) return rc;
+  if (int rc = check_h(x)) return rc;
+  if (h->n != x->n || h->dtype != x->dtype) return fail(IQSH_ERR_ARG, "registers differ in size or precision");
+  return IQSH_OK;
+}
+
+int iqsh_overlap(iqsh_handle h, iqsh_handle x, double* re, double* im)
+{
+  if (int rc = two_check(h, x)) return rc;
+  double r[2];
+  if (int rc = DISPATCH(h, do_two)(h, x, 0, 0, 0, r)) return rc;
+  if (re) *re = r[0];
+  if (im) *im = r[1];
+  return IQSH_OK;
+}
+
+int iqsh_l2diff2(iqsh_handle h, iqsh_handle x, double* out)
+{
+  if (int rc = two_check(h, x)) return rc;
+  if (!out) return fail(IQSH_ERR_ARG, "null output");
+  return DISPATCH(h, do_two)(h, x, 1, 0, 0, out);
+}
+
+int iqsh_maxabsdiff(iqsh_handle h, iqsh_handle x, double sr, double si, double* out)
+{
+  if (int rc = two_check(h, x)) return rc;
+  if (!out) return fail(IQSH_ERR_ARG, "null output");
+  return DISPATCH(h, do_two)(h, x, 2, sr, si, out);
+}
+
+// ---- QAOA ----
+namespace {
+unsigned stream_grid(uint64_t N) { return (unsigned)std::min<uint64_t>(grid_for(N, kBlock * 4), 8192); }
+
+template <typename T>
+int do_qaoa_phase(iqsh_handle h, const CutSpec& cs)
+{
+  k_qaoa_phase<T><<<stream_grid(h->size()), kBlock, 0, g_stream>>>((typename Vec<T>::t*)h->d, h->size(), cs);
+  CHECK_LAUNCH();
+  return IQSH_OK;
+}
+
+template <typename T>
+int do_qaoa_cost(iqsh_handle h, iqsh_handle lam, const CutSpec& cs, double* out)
+{
+  RedWork* w;
+  if (int rc = red_ws(&w)) return rc;
+  const unsigned g = red_grid(h->size());
+  k_qaoa_cost<T><<<g, kBlock, 0, g_stream>>>((const typename Vec<T>::t*)h->d,
+                                             lam ? (typename Vec<T>::t*)lam->d : nullptr, h->size(), cs, lam != nullptr,
+                                             w->partial);
+  CHECK_LAUNCH();
+  return red_finish(w, g, 1, 0, out);
+}
+
+template <typename T>
+int do_qaoa_phase_adj(iqsh_handle h, iqsh_handle lam, const CutSpec& cs, double* out)
+{
+  RedWork* w;
+  if (int rc = red_ws(&w)) return rc;
+  const unsigned g = red_grid(h->size());
+  k_qaoa_phase_adj<T><<<g, kBlock, 0, g_stream>>>((typename Vec<T>::t*)h->d, (typename Vec<T>::t*)lam->d, h->size(), cs,
+                                                  w->partial);
+  CHECK_LAUNCH();
+  double s;
+  if (int rc = red_finish(w, g, 1, 0, &s)) return rc;
+  *out = 2.0 * s;
+  return IQSH_OK;
+}
+
+template <typename T>
+int do_maxcut_fill(iqsh_handle h, const CutSpec& cs, double* out)
+{
+  RedWork* w;
+  if (int rc = red_ws(&w)) return rc;
+  const unsigned g = red_grid(h->size());
+  k_maxcut_fill<T><<<g, kBlock, 0, g_stream>>>((typename Vec<T>::t*)h->d, h->size(), cs, w->partial);
+  CHECK_LAUNCH();
+  return red_finish(w, g, 1, 1, out);
+}
+
+template <typename T>
+int do_qaoa_diag(iqsh_handle psi, iqsh_handle diag, double gamma)
+{
+  k_qaoa_diag<T><<<stream_grid(psi->size()), kBlock, 0, g_stream>>>((typename Vec<T>::t*)psi->d,
+                                                                    (const typename Vec<T>::t*)diag->d, psi->size(),
+                                                                    (T)gamma);
+  CHECK_LAUNCH();
+  return IQSH_OK;
+}
+
+template <typename T>
+int do_qaoa_diag_expect(iqsh_handle psi, iqsh_handle diag, int squared, double* out)
+{
+  RedWork* w;
+  if (int rc = red_ws(&w)) return rc;
+  const unsigned g = red_grid(psi->size());
+  k_qaoa_diag_expect<T><<<g, kBlock, 0, g_stream>>>((const typename Vec<T>::t*)psi->d,
+                                                    (const typename Vec<T>::t*)diag->d, psi->size(), squared, w->partial);
+  CHECK_LAUNCH();
+  return red_finish(w, g, 1, 0, out);
+}
+
+template <typename T>
+int do_adj_1q(iqsh_handle h, iqsh_handle lam, unsigned pos, const double* uinv, const double* G, double* out)
+{
+  RedWork* w;
+  if (int rc = red_ws(&w)) return rc;
+  const uint64_t items = h->size() / 2;
+  const unsigned g = red_grid(items);
+  const double zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  k_adj_1q<T><<<g, kBlock, 0, g_stream>>>((typename Vec<T>::t*)h->d, (typename Vec<T>::t*)lam->d, items, pos,
+                                          mat2<T>(uinv), mat2<double>(G ? G : zero), G != nullptr, w->partial);
+  CHECK_LAUNCH();
+  if (!G) return IQSH_OK;
+  double s;
+  if (int rc = red_finish(w, g, 1, 0, &s)) return rc;
+  *out = 2.0 * s;
+  return IQSH_OK;
+}
+}  // namespace
+
+int iqsh_qaoa_phase(iqsh_handle h, unsigned ne, const unsigned* u, const unsigned* v, const double* w, double gamma)
+{
+  if (int rc = check_h(h)) return rc;
+  if (ne && (!u || !v)) return fail(IQSH_ERR_ARG, "null edge list");
+  CutSpec cs;
+  if (int rc = build_cut(h, ne, u, v, w, gamma, &cs)) return rc;
+  return DISPATCH(h, do_qaoa_phase)(h, cs);
+}
+
+int iqsh_qaoa_expect(iqsh_handle h, unsigned ne, const unsigned* u, const unsigned* v, const double* w, double* out)
+{
+  if (int rc = check_h(h)) return rc;
+  if (!out || (ne && (!u || !v))) return fail(IQSH_ERR_ARG, "null argument");
+  CutSpec cs;
+  if (int rc = build_cut(h, ne, u, v, w, 0.0, &cs)) return rc;
+  return DISPATCH(h, do_qaoa_cost)(h, nullptr, cs, out);
+}
+
+int iqsh_qaoa_cost_apply(iqsh_handle psi, iqsh_handle lam, unsigned ne, const unsigned* u, const unsigned* v,
+                         const double* w, double* cost)
+{
+  if (int rc = two_check(psi, lam)) return rc;
+  if (!cost || (ne && (!u || !v))) return fail(IQSH_ERR_ARG, "null argument");
+  CutSpec cs;
+  if (int rc = build_cut(psi, ne, u, v, w, 0.0, &cs)) return rc;
+  return DISPATCH(psi, do_qaoa_cost)(psi, lam, cs, cost);
+}
+
+int iqsh_qaoa_phase_adjoint(iqsh_handle psi, iqsh_handle lam, unsigned ne, const unsigned* u, const unsigned* v,
+                            const double* w, double gamma, double* grad)
+{
+  if (int rc = two_check(psi, lam)) return rc;
+  if (!grad || (ne && (!u || !v))) return fail(IQSH_ERR_ARG, "null argument");
+  CutSpec cs;
+  // Un-applying exp(-i gamma C) is applying exp(-i (-gamma) C).
+  if (int rc = build_cut(psi, ne, u, v, w, -gamma, &cs)) return rc;
+  return DISPATCH(psi, do_qaoa_phase_adj)(psi, lam, cs, grad);
+}
+
+int iqsh_maxcut_diag(iqsh_handle diag, unsigned ne, const unsigned* u, const unsigned* v, const double* w,
+                     double* max_cut)
+{
+  if (int rc = check_h(diag)) return rc;
+  if (ne && (!u || !v)) return fail(IQSH_ERR_ARG, "null edge list");
+  CutSpec cs;
+  if (int rc = build_cut(diag, ne, u, v, w, 0.0, &cs)) return rc;
+  double m;
+  if (int rc = DISPATCH(diag, do_maxcut_fill)(diag, cs, &m)) return rc;
+  if (max_cut) *max_cut = m;
+  return IQSH_OK;
+}
+
+int iqsh_qaoa_phase_diag(iqsh_handle psi, iqsh_handle diag, double gamma)
+{
+  if (int rc = two_check(psi, diag)) return rc;
+  return DISPATCH(psi, do_qaoa_diag)(psi, diag, gamma);
+}
+
+int iqsh_qaoa_expect_diag(iqsh_handle psi, iqsh_handle diag, int squared, double* out)
+{
+  if (int rc = two_check(psi, diag)) return rc;
+  if (!out) return fail(IQSH_ERR_ARG, "null output");
+  return DISPATCH(psi, do_qaoa_diag_expect)(psi, diag, squared, out);
+}
+
+int iqsh_adjoint_1q(iqsh_handle psi, iqsh_handle lam, unsigned pos, const double uinv[8], const double* G, double* grad)
+{
+  if (int rc = two_check(psi, lam)) return rc;
+  if (int rc = check_pos(psi, pos)) return rc;
+  if (!uinv || (G && !grad)) return fail(IQSH_ERR_ARG, "null argument");
+  return DISPATCH(psi, do_adj_1q)(psi, lam, pos, uinv, G, grad);
+}

@@ -1,0 +1,380 @@
+// gfx950 kernels of the state-vector engine.  All kernels are HBM-streaming: every
+// amplitude a kernel touches is read once and written at most once per launch.
+//
+// Index scheme.  A gate on positions {p_0 < ... < p_{K-1}} touches 2^K amplitudes per
+// "item"; items are enumerated by k in [0, 2^(n-K)) and the item's base index is k with a
+// zero bit inserted at each p_s (ins0, ascending).  Consecutive threads take consecutive
+// k, so every load/store instruction of a wavefront covers one contiguous run of the
+// state whenever the lowest touched position is >= 6 (DP) / 7 (SP); for lower positions
+// the two partner loads of one thread land in the same cache lines and are merged in L2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace iqsk {
+
+constexpr int kBlock = 256;  // 4 wavefronts of 64
+
+// Amplitudes are clang ext-vector pairs (re, im): one 16-byte (DP) or 8-byte (SP) access.
+typedef double dcx __attribute__((ext_vector_type(2)));
+typedef float fcx __attribute__((ext_vector_type(2)));
+template <typename T> struct Vec;
+template <> struct Vec<double> { using t = dcx; };
+template <> struct Vec<float> { using t = fcx; };
+
+// Streaming accesses.  IQS_NT=1 (default) marks them non-temporal (the state is touched
+// once per launch); IQS_NT=0 uses ordinary cached accesses.
+#ifndef IQS_NT
+#define IQS_NT 1
+#endif
+template <typename V>
+__device__ __forceinline__ V ldg(const V* p)
+{
+#if IQS_NT
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+template <typename V>
+__device__ __forceinline__ void stg(const V& v, V* p)
+{
+#if IQS_NT
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+
+template <typename T> struct Cx { T r, i; };
+
+template <int K> struct Ins { unsigned p[K > 0 ? K : 1]; };
+
+template <typename T> struct Mat2 { Cx<T> m00, m01, m10, m11; };
+template <typename T> struct Mat4 { Cx<T> m[16]; };
+
+__device__ __forceinline__ uint64_t ins0(uint64_t x, unsigned p)
+{
+  const uint64_t lo = x & ((uint64_t(1) << p) - 1);
+  return ((x ^ lo) << 1) | lo;
+}
+
+template <int K>
+__device__ __forceinline__ uint64_t item_base(uint64_t k, const Ins<K>& ins)
+{
+#pragma unroll
+  for (int s = 0; s < K; ++s) k = ins0(k, ins.p[s]);
+  return k;
+}
+
+// a*x + b*y, written in the reference's operand order (m00*in0 + m01*in1).
+template <typename T, typename V>
+__device__ __forceinline__ V cmad2(const Cx<T>& a, const V& x, const Cx<T>& b, const V& y)
+{
+  V r;
+  r.x = (a.r * x.x - a.i * x.y) + (b.r * y.x - b.i * y.y);
+  r.y = (a.r * x.y + a.i * x.x) + (b.r * y.y + b.i * y.x);
+  return r;
+}
+
+template <typename T, typename V>
+__device__ __forceinline__ V cmul(const Cx<T>& a, const V& x)
+{
+  V r;
+  r.x = a.r * x.x - a.i * x.y;
+  r.y = a.r * x.y + a.i * x.x;
+  return r;
+}
+
+// ---------------------------------------------------------------------------------------
+// Pair kernel: (st[b|mask0], st[b|mask1]) <- M (st[b|mask0], st[b|mask1]) for every item.
+//   1q gate:          K=1 ins={t},        mask0=0,       mask1=1<<t
+//   controlled gate:  K=2 ins={c,t} sort, mask0=1<<c,    mask1=1<<c|1<<t
+//   multi-controlled: K=1+#c,             mask0=ctrl,    mask1=ctrl|1<<t
+//   swap-like:        K=2 ins={p1,p2},    mask0=1<<p1,   mask1=1<<p2
+// MOVE=true exchanges the two amplitudes (X-type matrices: a pure, bit-exact move).
+template <typename T, int K, int P, bool MOVE>
+__global__ __launch_bounds__(kBlock) void k_pair(typename Vec<T>::t* __restrict__ st, uint64_t nitems,
+                                                 Ins<K> ins, uint64_t mask0, uint64_t mask1, Mat2<T> m)
+{
+  using V = typename Vec<T>::t;
+  const uint64_t k0 = (uint64_t)blockIdx.x * (kBlock * P) + threadIdx.x;
+  V a[P], b[P];
+  uint64_t base[P];
+  if (k0 + (uint64_t)(P - 1) * kBlock < nitems) {
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      base[j] = item_base(k0 + (uint64_t)j * kBlock, ins);
+      a[j] = ldg(&st[base[j] | mask0]);
+      b[j] = ldg(&st[base[j] | mask1]);
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      if (MOVE) {
+        stg(b[j], &st[base[j] | mask0]);
+        stg(a[j], &st[base[j] | mask1]);
+      } else {
+        stg(cmad2(m.m00, a[j], m.m01, b[j]), &st[base[j] | mask0]);
+        stg(cmad2(m.m10, a[j], m.m11, b[j]), &st[base[j] | mask1]);
+      }
+    }
+  } else {
+    for (int j = 0; j < P; ++j) {
+      const uint64_t k = k0 + (uint64_t)j * kBlock;
+      if (k >= nitems) break;
+      const uint64_t bb = item_base(k, ins);
+      const V x = st[bb | mask0], y = st[bb | mask1];
+      if (MOVE) {
+        st[bb | mask0] = y;
+        st[bb | mask1] = x;
+      } else {
+        st[bb | mask0] = cmad2(m.m00, x, m.m01, y);
+        st[bb | mask1] = cmad2(m.m10, x, m.m11, y);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Scale kernel: st[b|mask] *= f for every item (K=0: the whole state).  Serves diagonal
+// gates whose other entries are exactly 1 (Z, S, T, CZ, CPhase, ...) by touching only the
+// amplitudes that change.
+template <typename T, int K, int P>
+__global__ __launch_bounds__(kBlock) void k_scale(typename Vec<T>::t* __restrict__ st, uint64_t nitems,
+                                                  Ins<K> ins, uint64_t mask, Cx<T> f)
+{
+  using V = typename Vec<T>::t;
+  const uint64_t k0 = (uint64_t)blockIdx.x * (kBlock * P) + threadIdx.x;
+  if (k0 + (uint64_t)(P - 1) * kBlock < nitems) {
+    V a[P];
+    uint64_t idx[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      idx[j] = item_base(k0 + (uint64_t)j * kBlock, ins) | mask;
+      a[j] = ldg(&st[idx[j]]);
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) stg(cmul(f, a[j]), &st[idx[j]]);
+  } else {
+    for (int j = 0; j < P; ++j) {
+      const uint64_t k = k0 + (uint64_t)j * kBlock;
+      if (k >= nitems) break;
+      const uint64_t i = item_base(k, ins) | mask;
+      st[i] = cmul(f, st[i]);
+    }
+  }
+}
+
+// Elementwise 2-qubit diagonal: st[i] *= d[2*b(p1) + b(p2)].
+template <typename T, int P>
+__global__ __launch_bounds__(kBlock) void k_diag2(typename Vec<T>::t* __restrict__ st, uint64_t N, unsigned p1,
+                                                  unsigned p2, Cx<T> d0, Cx<T> d1, Cx<T> d2, Cx<T> d3)
+{
+  using V = typename Vec<T>::t;
+  const uint64_t i0 = (uint64_t)blockIdx.x * (kBlock * P) + threadIdx.x;
+  V a[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    const uint64_t i = i0 + (uint64_t)j * kBlock;
+    if (i < N) a[j] = ldg(&st[i]);
+  }
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    const uint64_t i = i0 + (uint64_t)j * kBlock;
+    if (i < N) {
+      const bool b1 = (i >> p1) & 1, b2 = (i >> p2) & 1;
+      const Cx<T> d = b1 ? (b2 ? d3 : d2) : (b2 ? d1 : d0);
+      stg(cmul(d, a[j]), &st[i]);
+    }
+  }
+}
+
+// Dense 4x4 on (ph, pl): indices b, b|dl, b|dh, b|dh|dl; new[t] = sum_c m[t][c] old[c].
+template <typename T, int P>
+__global__ __launch_bounds__(kBlock) void k_quad(typename Vec<T>::t* __restrict__ st, uint64_t nitems,
+                                                 Ins<2> ins, uint64_t dl, uint64_t dh, Mat4<T> m)
+{
+  using V = typename Vec<T>::t;
+  const uint64_t k0 = (uint64_t)blockIdx.x * (kBlock * P) + threadIdx.x;
+  V x[P][4];
+  uint64_t base[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    const uint64_t k = k0 + (uint64_t)j * kBlock;
+    base[j] = item_base(k, ins);
+    if (k < nitems) {
+      x[j][0] = ldg(&st[base[j]]);
+      x[j][1] = ldg(&st[base[j] | dl]);
+      x[j][2] = ldg(&st[base[j] | dh]);
+      x[j][3] = ldg(&st[base[j] | dh | dl]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    const uint64_t k = k0 + (uint64_t)j * kBlock;
+    if (k >= nitems) continue;
+    const uint64_t off[4] = {0, dl, dh, dh | dl};
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      V r = cmad2(m.m[4 * t + 0], x[j][0], m.m[4 * t + 1], x[j][1]);
+      const V s = cmad2(m.m[4 * t + 2], x[j][2], m.m[4 * t + 3], x[j][3]);
+      r.x += s.x;
+      r.y += s.y;
+      stg(r, &st[base[j] | off[t]]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Dense 2^KQ block on arbitrary positions.  A block stages G groups of 2^KQ amplitudes in
+// LDS (gathered through the position masks), then each thread produces one output
+// amplitude of one group from the matrix held in LDS.
+template <typename T, int KQ>
+__global__ __launch_bounds__(kBlock) void k_block(typename Vec<T>::t* __restrict__ st, uint64_t ngroups,
+                                                  Ins<KQ> ins, const typename Vec<T>::t* __restrict__ U)
+{
+  using V = typename Vec<T>::t;
+  constexpr int D = 1 << KQ;
+  constexpr int G = kBlock / D;  // groups per block
+  __shared__ V sU[D * D];
+  __shared__ V sx[G][D];
+  for (int e = threadIdx.x; e < D * D; e += kBlock) sU[e] = U[e];
+  // Per-row offsets: bit j of the row index sets position ins.p[j] (ins sorted ascending,
+  // and the host orders the matrix rows accordingly).
+  const int g = threadIdx.x / D, r = threadIdx.x % D;
+  uint64_t off = 0;
+#pragma unroll
+  for (int j = 0; j < KQ; ++j)
+    if ((r >> j) & 1) off |= uint64_t(1) << ins.p[j];
+  const uint64_t grp = (uint64_t)blockIdx.x * G + g;
+  uint64_t idx = 0;
+  if (grp < ngroups) {
+    idx = item_base(grp, ins) | off;
+    sx[g][r] = st[idx];
+  }
+  __syncthreads();
+  if (grp < ngroups) {
+    T yr = 0, yi = 0;
+#pragma unroll 8
+    for (int c = 0; c < D; ++c) {
+      const V u = sU[r * D + c], x = sx[g][c];
+      yr += u.x * x.x - u.y * x.y;
+      yi += u.x * x.y + u.y * x.x;
+    }
+    V y;
+    y.x = yr;
+    y.y = yi;
+    st[idx] = y;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Fills.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill(typename Vec<T>::t* __restrict__ st, uint64_t N, Cx<T> v)
+{
+  using V = typename Vec<T>::t;
+  V a;
+  a.x = v.r;
+  a.y = v.i;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (uint64_t)gridDim.x * kBlock)
+    stg(a, &st[i]);
+}
+
+// h += f * x
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_axpy(typename Vec<T>::t* __restrict__ h, const typename Vec<T>::t* __restrict__ x,
+                                                 uint64_t N, Cx<T> f)
+{
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (uint64_t)gridDim.x * kBlock) {
+    auto a = h[i];
+    const auto b = cmul(f, x[i]);
+    a.x += b.x;
+    a.y += b.y;
+    h[i] = a;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Byte-table index transforms.  A 64-bit value f(x) that is a XOR (or OR) of per-bit
+// contributions c[p] over the set bits p of x is evaluated as the XOR of up to 8 table
+// lookups T[byte][x's byte], the tables built in LDS by each block from c[] (kernarg).
+
+struct BitTab {
+  uint64_t c[64];   // contribution of each index bit
+  unsigned nbytes;  // number of index bytes in use (ceil(n / 8))
+};
+
+__device__ __forceinline__ void build_tables(uint64_t (*tab)[256], const BitTab& bt)
+{
+  for (unsigned e = threadIdx.x; e < bt.nbytes * 256; e += kBlock) {
+    const unsigned byte = e >> 8, v = e & 255;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if ((v >> k) & 1) acc ^= bt.c[byte * 8 + k];
+    tab[byte][v] = acc;
+  }
+}
+
+__device__ __forceinline__ uint64_t eval_tables(const uint64_t (*tab)[256], unsigned nbytes, uint64_t x)
+{
+  uint64_t acc = 0;
+  for (unsigned b = 0; b < nbytes; ++b) acc ^= tab[b][(x >> (8 * b)) & 255];
+  return acc;
+}
+
+// Permutation: new[j] = old[src(j)], src(j) = OR of (bit b of j) << src_pos[b].
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_permute(typename Vec<T>::t* __restrict__ dst,
+                                                    const typename Vec<T>::t* __restrict__ src, uint64_t N, BitTab bt)
+{
+  __shared__ uint64_t tab[8][256];
+  build_tables(tab, bt);
+  __syncthreads();
+  for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < N; j += (uint64_t)gridDim.x * kBlock)
+    stg(src[eval_tables(tab, bt.nbytes, j)], &dst[j]);
+}
+
+// ---------------------------------------------------------------------------------------
+// QAOA MaxCut: edge parity vector E(x) (bit e = [x_u != x_v]) from incidence tables; the cut
+// is popcount(E) (unit weights) or sum of weights over set bits of E (byte tables).
+struct CutSpec {
+  BitTab inc;         // inc.c[p] = mask of edges incident to data position p
+  double w[64];       // edge weights
+  unsigned ne;        // number of edges (<= 64)
+  int weighted;       // 0: integer cut -> phase table; 1: weighted cut
+  double ph_re[65];   // unit weights: exp(-i gamma c) for c = 0..ne (host-computed)
+  double ph_im[65];
+  double gamma;
+};
+
+__device__ __forceinline__ void build_weight_tables(double (*wt)[256], const CutSpec& cs)
+{
+  const unsigned nb = (cs.ne + 7) / 8;
+  for (unsigned e = threadIdx.x; e < nb * 256; e += kBlock) {
+    const unsigned byte = e >> 8, v = e & 255;
+    double acc = 0;
+    for (int k = 0; k < 8; ++k)
+      if (((v >> k) & 1) && byte * 8 + k < cs.ne) acc += cs.w[byte * 8 + k];
+    wt[byte][v] = acc;
+  }
+}
+
+__device__ __forceinline__ double cut_value(const uint64_t (*tab)[256], const double (*wt)[256], const CutSpec& cs,
+                                            uint64_t x, int* icut)
+{
+  const uint64_t E = eval_tables(tab, cs.inc.nbytes, x);
+  if (!cs.weighted) {
+    *icut = __popcll(E);
+    return (double)*icut;
+  }
+  double c = 0;
+  const unsigned nb = (cs.ne + 7) / 8;
+  for (unsigned b = 0; b < nb; ++b) c += wt[b][(E >> (8 * b)) & 255];
+  *icut = 0;
+  return c;
+}
+
+}  // namespace iqsk

@@ -1,0 +1,309 @@
+"""Kernel-level parity on the GPU: every C-ABI entry point against the CPU oracle on the
+same seeded input.  Tolerances: 1e-12 max-abs (complex128), 1e-5 (complex64); moves and
+permutations bit-exact; reductions against FP64 sums."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from iqs_amd.circuits import SplitMix64, haar_unitary
+from iqs_amd.engine import StateVector
+
+pytestmark = pytest.mark.gpu
+
+DT = {"c128": np.complex128, "c64": np.complex64}
+TOL = {"c128": 1e-12, "c64": 1e-5}
+
+
+def rand_state(n, dt, seed):
+    r = np.random.default_rng(seed)
+    v = r.normal(size=1 << n) + 1j * r.normal(size=1 << n)
+    return (v / np.linalg.norm(v)).astype(DT[dt])
+
+
+def on_gpu(host, dt):
+    sv = StateVector(int(host.size).bit_length() - 1, dt)
+    sv.upload(host)
+    return sv
+
+
+@pytest.fixture(params=["c128", "c64"])
+def dt(request):
+    return request.param
+
+
+def test_upload_download_roundtrip(dt):
+    h = rand_state(12, dt, 1)
+    sv = on_gpu(h, dt)
+    assert np.array_equal(sv.download(), h)
+    assert np.array_equal(sv.download(100, 7), h[100:107])
+
+
+@pytest.mark.parametrize("pos", range(12))
+def test_1q_dense_every_position(dt, pos):
+    n = 12
+    m = haar_unitary(2, SplitMix64(pos + 11))
+    h = rand_state(n, dt, pos)
+    sv = on_gpu(h, dt)
+    sv.apply_1q(pos, m)
+    oracle.apply_1q(h, pos, m)
+    assert np.abs(sv.download() - h).max() < TOL[dt]
+
+
+@pytest.mark.parametrize("kind", ["X", "Y", "Z", "S", "T", "RZ", "H", "I"])
+def test_1q_structured(dt, kind):
+    n, pos = 11, 5
+    mats = {
+        "X": [[0, 1], [1, 0]], "Y": [[0, -1j], [1j, 0]], "Z": [[1, 0], [0, -1]],
+        "S": [[1, 0], [0, 1j]], "T": [[1, 0], [0, np.exp(1j * math.pi / 4)]],
+        "RZ": [[np.exp(-0.3j), 0], [0, np.exp(0.3j)]],
+        "H": np.array([[1, 1], [1, -1]]) / math.sqrt(2), "I": [[1, 0], [0, 1]],
+    }
+    m = np.array(mats[kind], dtype=complex)
+    h = rand_state(n, dt, 3)
+    sv = on_gpu(h, dt)
+    sv.apply_1q(pos, m)
+    oracle.apply_1q(h, pos, m)
+    got = sv.download()
+    if kind in ("X", "I"):
+        assert np.array_equal(got, h)  # pure move / no-op
+    else:
+        assert np.abs(got - h).max() < TOL[dt]
+
+
+@pytest.mark.parametrize("c,t", [(0, 1), (1, 0), (0, 10), (10, 0), (3, 7), (7, 3), (9, 10)])
+def test_c1q(dt, c, t):
+    n = 11
+    m = haar_unitary(2, SplitMix64(c * 13 + t))
+    h = rand_state(n, dt, c + t)
+    sv = on_gpu(h, dt)
+    sv.apply_c1q(c, t, m)
+    oracle.apply_c1q(h, c, t, m)
+    assert np.abs(sv.download() - h).max() < TOL[dt]
+
+
+def test_cnot_and_cz_exact(dt):
+    n = 10
+    h = rand_state(n, dt, 9)
+    sv = on_gpu(h, dt)
+    X = np.array([[0, 1], [1, 0]], dtype=complex)
+    Z = np.array([[1, 0], [0, -1]], dtype=complex)
+    for (c, t) in [(0, 1), (5, 2), (9, 0)]:
+        sv.apply_c1q(c, t, X)
+        oracle.apply_c1q(h, c, t, X)
+    assert np.array_equal(sv.download(), h)
+    sv.apply_c1q(3, 8, Z)
+    oracle.apply_c1q(h, 3, 8, Z)
+    assert np.array_equal(sv.download(), h)
+
+
+def test_multi_controlled_toffoli(dt):
+    n = 9
+    h = rand_state(n, dt, 21)
+    sv = on_gpu(h, dt)
+    X = np.array([[0, 1], [1, 0]], dtype=complex)
+    sv.apply_mc1q([2, 7], 4, X)
+    want = h.copy()
+    for i in range(1 << n):
+        if (i >> 2) & 1 and (i >> 7) & 1 and not (i >> 4) & 1:
+            j = i | (1 << 4)
+            want[i], want[j] = h[j], h[i]
+    assert np.array_equal(sv.download(), want)
+
+
+@pytest.mark.parametrize("p1,p2", [(0, 1), (1, 0), (0, 10), (4, 6), (10, 3)])
+def test_swaplike(dt, p1, p2):
+    n = 11
+    h = rand_state(n, dt, 5)
+    sv = on_gpu(h, dt)
+    th = 0.37 + p1
+    m = np.array([[math.cos(th), 1j * math.sin(th)], [1j * math.sin(th), math.cos(th)]])
+    sv.apply_swaplike(p1, p2, m)
+    oracle.apply_swaplike(h, min(p1, p2), max(p1, p2), m)
+    assert np.abs(sv.download() - h).max() < TOL[dt]
+    sv.apply_swaplike(p1, p2, [[0, 1], [1, 0]])
+    oracle.apply_swaplike(h, min(p1, p2), max(p1, p2), np.array([[0, 1], [1, 0]]))
+    assert np.abs(sv.download() - h).max() < TOL[dt]
+
+
+@pytest.mark.parametrize("p1,p2", [(0, 1), (1, 0), (2, 9), (9, 2)])
+@pytest.mark.parametrize("kind", ["general", "cz", "cphase01"])
+def test_diag2(dt, p1, p2, kind):
+    n = 10
+    d = {"general": np.exp(1j * np.array([0.2, -1.1, 0.7, 2.5])),
+         "cz": np.array([1, 1, 1, -1], dtype=complex),
+         "cphase01": np.array([1, np.exp(0.4j), 1, 1])}[kind]
+    h = rand_state(n, dt, 8)
+    sv = on_gpu(h, dt)
+    sv.apply_diag2(p1, p2, d)
+    oracle.apply_diag(h, p1, p2, d)
+    assert np.abs(sv.download() - h).max() < TOL[dt]
+
+
+@pytest.mark.parametrize("ph,pl", [(1, 0), (0, 1), (9, 0), (0, 9), (5, 6), (8, 3)])
+def test_2q_dense(dt, ph, pl):
+    n = 10
+    m = haar_unitary(4, SplitMix64(ph * 17 + pl))
+    h = rand_state(n, dt, ph + 2 * pl)
+    sv = on_gpu(h, dt)
+    sv.apply_2q(ph, pl, m)
+    oracle.apply_2q(h, ph, pl, m)
+    assert np.abs(sv.download() - h).max() < TOL[dt]
+
+
+@pytest.mark.parametrize("positions", [[0], [3], [0, 1], [7, 2], [0, 1, 2], [9, 4, 0], [1, 3, 5, 7],
+                                       [0, 1, 2, 3, 4], [9, 2, 6, 0, 4]])
+def test_kq_block_matches_kronecker(dt, positions):
+    n = 10
+    k = len(positions)
+    U = haar_unitary(1 << k, SplitMix64(sum(positions) + 100 * k))
+    h = rand_state(n, dt, k)
+    sv = on_gpu(h, dt)
+    sv.apply_kq(positions, U)
+    # reference: explicit gather/scatter with row bit j <-> positions[j]
+    want = h.astype(np.complex128).copy()
+    mask = sum(1 << p for p in positions)
+    for base in range(1 << n):
+        if base & mask:
+            continue
+        idx = [base | sum(((r >> j) & 1) << positions[j] for j in range(k)) for r in range(1 << k)]
+        want[idx] = U @ h.astype(np.complex128)[idx]
+    assert np.abs(sv.download() - want).max() < (1e-12 if dt == "c128" else 2e-5)
+
+
+@pytest.mark.parametrize("perm_seed", [1, 2, 3])
+def test_permute_bit_exact(dt, perm_seed):
+    n = 11
+    r = np.random.default_rng(perm_seed)
+    src = list(r.permutation(n))
+    h = rand_state(n, dt, perm_seed)
+    sv = on_gpu(h, dt)
+    sv.permute(src)
+    j = np.arange(1 << n)
+    srcidx = np.zeros_like(j)
+    for b in range(n):
+        srcidx |= ((j >> b) & 1) << src[b]
+    assert np.array_equal(sv.download(), h[srcidx])
+
+
+def test_reductions(dt):
+    n = 12
+    h = rand_state(n, dt, 31) * 1.7
+    sv = on_gpu(h, dt)
+    a = h.astype(np.complex128)
+    p = np.abs(a) ** 2
+    idx = np.arange(1 << n)
+    for pos in (0, 5, 11):
+        assert abs(sv.prob1(pos) - p[(idx >> pos) & 1 == 1].sum()) < 1e-12
+    assert abs(sv.norm2() - p.sum()) < 1e-12
+    g = rand_state(n, dt, 32)
+    sg = on_gpu(g, dt)
+    assert abs(sv.overlap(sg) - np.vdot(g.astype(np.complex128), a)) < 1e-12
+    assert abs(sv.l2diff2(sg) - (np.abs(a - g) ** 2).sum()) < 1e-12
+    gg = g.astype(np.complex128)
+    assert abs(sv.maxabsdiff(sg, 0.5 - 0.25j) - np.abs(a - (0.5 - 0.25j) * gg).max()) < 1e-12
+
+
+def pauli_dense(n, paulis):
+    """Explicit operator of a Pauli string {position: 'X'|'Y'|'Z'}."""
+    P = {"I": np.eye(2), "X": np.array([[0, 1], [1, 0]]), "Y": np.array([[0, -1j], [1j, 0]]),
+         "Z": np.diag([1, -1])}
+    full = np.array([[1.0 + 0j]])
+    for q in reversed(range(n)):
+        full = np.kron(full, P[paulis.get(q, "I")])
+    return full
+
+
+@pytest.mark.parametrize("paulis", [{0: "Z"}, {3: "X"}, {2: "Y"}, {0: "X", 1: "X"}, {1: "Y", 4: "Y"},
+                                    {0: "X", 2: "Y", 5: "Z"}, {5: "Y", 3: "Y", 1: "Y"}, {4: "Z", 1: "Z"}])
+def test_pauli_expectation_one_pass(dt, paulis):
+    n = 6
+    h = rand_state(n, dt, 77) * 1.1
+    sv = on_gpu(h, dt)
+    x = sum(1 << q for q, s in paulis.items() if s in "XY")
+    z = sum(1 << q for q, s in paulis.items() if s in "YZ")
+    val, nn = sv.expval_pauli(x, z)
+    a = h.astype(np.complex128)
+    want = np.vdot(a, pauli_dense(n, paulis) @ a).real
+    assert abs(val - want) < 1e-12
+    assert abs(nn - np.vdot(a, a).real) < 1e-12
+    assert np.array_equal(sv.download(), h)  # read-only
+
+
+def test_scale_axpy_collapse_fill(dt):
+    n = 10
+    h = rand_state(n, dt, 4)
+    g = rand_state(n, dt, 5)
+    sv, sg = on_gpu(h, dt), on_gpu(g, dt)
+    sv.scale(0.3 - 0.2j)
+    sv.axpy(sg, 1.5j)
+    want = (0.3 - 0.2j) * h.astype(complex) + 1.5j * g.astype(complex)
+    assert np.abs(sv.download() - want).max() < 10 * TOL[dt]
+    sv.collapse(3, 1)
+    want[((np.arange(1 << n) >> 3) & 1) == 0] = 0
+    assert np.abs(sv.download() - want).max() < 10 * TOL[dt]
+    sv.init_uniform(0.25)
+    assert np.all(sv.download() == 0.25)
+    sv.init_basis(77)
+    d = sv.download()
+    assert d[77] == 1 and np.count_nonzero(d) == 1
+
+
+def edges_3reg(nv, seed):
+    from iqs_amd.circuits import random_regular_graph
+    return random_regular_graph(nv, 3, seed)
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_qaoa_phase_and_expectation(dt, weighted):
+    n = 10
+    edges = edges_3reg(n, 7)
+    if weighted:
+        edges = [(u, v, 0.5 + 0.1 * k) for k, (u, v, _) in enumerate(edges)]
+    h = rand_state(n, dt, 12)
+    sv = on_gpu(h, dt)
+    idx = np.arange(1 << n)
+    cut = np.zeros(1 << n)
+    for (u, v, w) in edges:
+        cut += w * (((idx >> u) ^ (idx >> v)) & 1)
+    gamma = 0.731
+    sv.qaoa_phase(edges, gamma)
+    want = h.astype(np.complex128) * np.exp(-1j * gamma * cut)
+    got = sv.download().astype(np.complex128)
+    assert np.abs(got - want).max() < TOL[dt]
+    assert abs(sv.qaoa_expect(edges) - (cut * np.abs(got) ** 2).sum()) < 1e-12
+    diag = StateVector(n, dt)
+    mx = diag.maxcut_diag(edges)
+    assert abs(mx - cut.max()) < 1e-12
+    assert np.abs(diag.download().real - cut).max() < 1e-5
+
+
+def test_qaoa_diag_register_matches_oracle(dt):
+    n = 8
+    edges = edges_3reg(n, 3)
+    h = rand_state(n, dt, 2)
+    sv = on_gpu(h, dt)
+    diag = StateVector(n, dt)
+    diag.maxcut_diag(edges)
+    dh = diag.download()
+    sv.qaoa_phase_diag(diag, 0.4)
+    oracle.qaoa_phase(h, dh, 0.4)
+    assert np.abs(sv.download() - h).max() < TOL[dt]
+    assert abs(sv.qaoa_expect_diag(diag) - oracle.qaoa_expect(h, dh)) < (1e-12 if dt == "c128" else 1e-5)
+
+
+def test_errors_are_reported_not_fatal():
+    from iqs_amd._lib import IqsError
+    sv = StateVector(4, "c128")
+    with pytest.raises(IqsError, match="out of range"):
+        sv.apply_1q(4, np.eye(2))
+    with pytest.raises(IqsError, match="coincide"):
+        sv.apply_c1q(1, 1, np.eye(2))
+    with pytest.raises(IqsError):
+        StateVector(41, "c128")
+    with pytest.raises(IqsError, match="not a permutation"):
+        sv.permute([0, 0, 1, 2])
+    other = StateVector(5, "c128")
+    with pytest.raises(IqsError, match="differ"):
+        sv.overlap(other)

@@ -151,18 +151,62 @@ bool is_c(const double* a, double re, double im) { return a[0] == re && a[1] == 
 // ---------------------------------------------------------------------------------------
 // Launchers
 
-constexpr int kP = 4;  // items per thread in streaming kernels
+// Launch configurations (see the tiling notes in iqs_kernels.hpp).
+constexpr int kTB = 1024, kTP = 2;  // tiled pair / scale / diag kernels
+constexpr int kLB = 256, kLP = 2;   // lane-shuffle 1q kernel
+constexpr int kSB = 256, kSP = 4;   // plain kernels for small states
+
+template <typename T, int K, bool MOVE>
+int launch_pair_m(iqsh_handle h, const Ins<K>& ins, uint64_t mask0, uint64_t mask1, const double* m)
+{
+  using V = typename Vec<T>::t;
+  const uint64_t items = h->size() >> K;
+  if (items >= kTiledMin)
+    k_pair<T, K, MOVE, true, kTB, kTP><<<(unsigned)(items / (kTB * kTP)), kTB, 0, g_stream>>>((V*)h->d, items, ins, mask0,
+                                                                                                 mask1, mat2<T>(m));
+  else
+    k_pair<T, K, MOVE, false, kSB, kSP><<<grid_for(items, kSB * kSP), kSB, 0, g_stream>>>((V*)h->d, items, ins, mask0,
+                                                                                          mask1, mat2<T>(m));
+  CHECK_LAUNCH();
+  return IQSH_OK;
+}
 
 template <typename T, int K>
 int launch_pair(iqsh_handle h, const Ins<K>& ins, uint64_t mask0, uint64_t mask1, const double* m, bool move)
 {
+  return move ? launch_pair_m<T, K, true>(h, ins, mask0, mask1, m) : launch_pair_m<T, K, false>(h, ins, mask0, mask1, m);
+}
+
+template <typename T, int KC, bool COND, bool MOVE>
+int launch_lane(iqsh_handle h, unsigned t, const Ins<KC>& cins, uint64_t ctrl, const double* m)
+{
   using V = typename Vec<T>::t;
-  const uint64_t items = h->size() >> K;
-  const unsigned grid = grid_for(items, kBlock * kP);
+  const uint64_t items = h->size() >> KC;
+  k_pair_lane<T, KC, COND, MOVE, kLB, kLP><<<(unsigned)(items / (kLB * kLP)), kLB, 0, g_stream>>>((V*)h->d, t, cins, ctrl,
+                                                                                                  mat2<T>(m));
+  CHECK_LAUNCH();
+  return IQSH_OK;
+}
+
+template <typename T, int KC>
+int lane_dispatch(iqsh_handle h, unsigned t, const Ins<KC>& cins, uint64_t ctrl, const double* m, bool move, bool cond)
+{
+  if (cond) return move ? launch_lane<T, KC, true, true>(h, t, cins, ctrl, m) : launch_lane<T, KC, true, false>(h, t, cins, ctrl, m);
+  return move ? launch_lane<T, KC, false, true>(h, t, cins, ctrl, m) : launch_lane<T, KC, false, false>(h, t, cins, ctrl, m);
+}
+
+template <typename T>
+int launch_pair_cond(iqsh_handle h, unsigned t, uint64_t ctrl, const double* m, bool move)
+{
+  using V = typename Vec<T>::t;
+  Ins<1> ins;
+  ins.p[0] = t;
+  const uint64_t items = h->size() >> 1;
+  const unsigned grid = (unsigned)(items / (kTB * kTP));
   if (move)
-    k_pair<T, K, kP, true><<<grid, kBlock, 0, g_stream>>>((V*)h->d, items, ins, mask0, mask1, mat2<T>(m));
+    k_pair_cond<T, true, kTB, kTP><<<grid, kTB, 0, g_stream>>>((V*)h->d, ins, ctrl, mat2<T>(m));
   else
-    k_pair<T, K, kP, false><<<grid, kBlock, 0, g_stream>>>((V*)h->d, items, ins, mask0, mask1, mat2<T>(m));
+    k_pair_cond<T, false, kTB, kTP><<<grid, kTB, 0, g_stream>>>((V*)h->d, ins, ctrl, mat2<T>(m));
   CHECK_LAUNCH();
   return IQSH_OK;
 }
@@ -172,7 +216,24 @@ int launch_scale(iqsh_handle h, const Ins<K>& ins, uint64_t mask, const double* 
 {
   using V = typename Vec<T>::t;
   const uint64_t items = h->size() >> K;
-  k_scale<T, K, kP><<<grid_for(items, kBlock * kP), kBlock, 0, g_stream>>>((V*)h->d, items, ins, mask, cx<T>(f));
+  if (items >= kTiledMin)
+    k_scale<T, K, true, kTB, kTP><<<(unsigned)(items / (kTB * kTP)), kTB, 0, g_stream>>>((V*)h->d, items, ins, mask,
+                                                                                          cx<T>(f));
+  else
+    k_scale<T, K, false, kSB, kSP><<<grid_for(items, kSB * kSP), kSB, 0, g_stream>>>((V*)h->d, items, ins, mask, cx<T>(f));
+  CHECK_LAUNCH();
+  return IQSH_OK;
+}
+
+template <typename T>
+int launch_diag2(iqsh_handle h, unsigned p1, unsigned p2, Cx<T> d0, Cx<T> d1, Cx<T> d2, Cx<T> d3)
+{
+  using V = typename Vec<T>::t;
+  const uint64_t N = h->size();
+  if (N >= kTiledMin)
+    k_diag2<T, true, kTB, kTP><<<(unsigned)(N / (kTB * kTP)), kTB, 0, g_stream>>>((V*)h->d, N, p1, p2, d0, d1, d2, d3);
+  else
+    k_diag2<T, false, kSB, kSP><<<grid_for(N, kSB * kSP), kSB, 0, g_stream>>>((V*)h->d, N, p1, p2, d0, d1, d2, d3);
   CHECK_LAUNCH();
   return IQSH_OK;
 }
@@ -232,11 +293,7 @@ int controlled_2x2(iqsh_handle h, uint64_t ctrl, unsigned t, const double* m)
     if (one0 && one1) return IQSH_OK;
     if (!one0 && !one1 && ctrl == 0) {
       // diag(a, b) on one position: one elementwise pass, d[2*b(t) + b(t)] in {a, b}.
-      using V = typename Vec<T>::t;
-      k_diag2<T, kP><<<grid_for(h->size(), kBlock * kP), kBlock, 0, g_stream>>>(
-          (V*)h->d, h->size(), t, t, cx<T>(m), Cx<T>{0, 0}, Cx<T>{0, 0}, cx<T>(m + 6));
-      CHECK_LAUNCH();
-      return IQSH_OK;
+      return launch_diag2<T>(h, t, t, cx<T>(m), Cx<T>{0, 0}, Cx<T>{0, 0}, cx<T>(m + 6));
     }
     int rc = IQSH_OK;
     if (!one1) rc = scale_dispatch<T>(h, ins, ctrl | tb, m + 6);
@@ -244,6 +301,27 @@ int controlled_2x2(iqsh_handle h, uint64_t ctrl, unsigned t, const double* m)
     return rc;
   }
   const bool xmove = is_c(m, 0, 0) && is_c(m + 6, 0, 0) && is_c(m + 2, 1, 0) && is_c(m + 4, 1, 0);
+  const int nc = __builtin_popcountll(ctrl);
+  const unsigned lowc = ctrl ? (unsigned)__builtin_ctzll(ctrl) : 64u;
+  if (h->size() >= (kTiledMin << 3)) {
+    if (t < 6) {
+      // Target inside a wavefront: lane-shuffle kernels.
+      if (lowc >= 6) {
+        std::vector<unsigned> cpos(ins.begin(), ins.end() - 1);
+        switch (nc) {
+          case 0: return lane_dispatch<T, 0>(h, t, Ins<0>{}, 0, m, xmove, false);
+          case 1: return lane_dispatch<T, 1>(h, t, sorted_ins<1>(cpos.data()), ctrl, m, xmove, false);
+          case 2: return lane_dispatch<T, 2>(h, t, sorted_ins<2>(cpos.data()), ctrl, m, xmove, false);
+          case 3: return lane_dispatch<T, 3>(h, t, sorted_ins<3>(cpos.data()), ctrl, m, xmove, false);
+          default: break;
+        }
+      } else {
+        return lane_dispatch<T, 0>(h, t, Ins<0>{}, ctrl, m, xmove, true);
+      }
+    } else if (lowc < 3) {
+      return launch_pair_cond<T>(h, t, ctrl, m, xmove);
+    }
+  }
   return pair_dispatch<T>(h, ins, ctrl, ctrl | tb, m, xmove);
 }
 
@@ -911,11 +989,7 @@ int do_diag2(iqsh_handle h, unsigned p1, unsigned p2, const double* d)
     const unsigned ins[2] = {p1, p2};
     return launch_scale<T, 2>(h, sorted_ins<2>(ins), mask, d + 2 * which);
   }
-  using V = typename Vec<T>::t;
-  k_diag2<T, kP><<<grid_for(h->size(), kBlock * kP), kBlock, 0, g_stream>>>(
-      (V*)h->d, h->size(), p1, p2, cx<T>(d), cx<T>(d + 2), cx<T>(d + 4), cx<T>(d + 6));
-  CHECK_LAUNCH();
-  return IQSH_OK;
+  return launch_diag2<T>(h, p1, p2, cx<T>(d), cx<T>(d + 2), cx<T>(d + 4), cx<T>(d + 6));
 }
 }  // namespace
 
@@ -938,8 +1012,12 @@ int do_2q(iqsh_handle h, unsigned ph, unsigned pl, const double* m)
   for (int i = 0; i < 16; ++i) M.m[i] = cx<T>(m + 2 * i);
   const unsigned ins[2] = {ph, pl};
   const uint64_t items = h->size() >> 2;
-  k_quad<T, 2><<<grid_for(items, kBlock * 2), kBlock, 0, g_stream>>>((V*)h->d, items, sorted_ins<2>(ins),
-                                                                      uint64_t(1) << pl, uint64_t(1) << ph, M);
+  if (items >= kTiledMin)
+    k_quad<T, true, kTB, 1><<<(unsigned)(items / kTB), kTB, 0, g_stream>>>((V*)h->d, items, sorted_ins<2>(ins),
+                                                                           uint64_t(1) << pl, uint64_t(1) << ph, M);
+  else
+    k_quad<T, false, kSB, 2><<<grid_for(items, kSB * 2), kSB, 0, g_stream>>>((V*)h->d, items, sorted_ins<2>(ins),
+                                                                            uint64_t(1) << pl, uint64_t(1) << ph, M);
   CHECK_LAUNCH();
   return IQSH_OK;
 }

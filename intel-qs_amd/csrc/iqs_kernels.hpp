@@ -68,6 +68,35 @@ __device__ __forceinline__ uint64_t item_base(uint64_t k, const Ins<K>& ins)
   return k;
 }
 
+// ---------------------------------------------------------------------------------------
+// Work tiling of the streaming kernels (measured on MI355X, 30-qubit complex128 state).
+//  * XCD-contiguous blocks: workgroups are dispatched round-robin over the 8 XCDs, so
+//    logical block (b % 8) * (G / 8) + b / 8 gives every XCD one contiguous slice of the
+//    item space instead of a 1-in-8 interleave.
+//  * Split blocks: a block of B*P items takes SUB sub-ranges of B*P/SUB items spaced
+//    2^kSubLog items apart.  Keeping a block's partner streams a few tens of KiB apart is
+//    what lets the pair update run at ~6.3 TB/s at every target position (a single
+//    contiguous range per block stays near 5.6-5.9 TB/s for far-apart partners).
+// Both need a power-of-two item count >= kTiledMin; smaller states use the plain mapping.
+constexpr unsigned kSubLog = 11;
+constexpr int kSub = 4;
+constexpr uint64_t kTiledMin = uint64_t(1) << 14;
+
+__device__ __forceinline__ uint64_t xcd_block(uint64_t b, uint64_t G) { return (b & 7) * (G >> 3) + (b >> 3); }
+
+// Item index of (block bb, slot t in [0, B*P)) under the split-block tiling.
+template <int B, int P, int SUB>
+__device__ __forceinline__ uint64_t tiled_item(uint64_t bb, unsigned t)
+{
+  if (SUB == 1) return bb * (uint64_t)(B * P) + t;
+  constexpr uint64_t sublen = (uint64_t)B * P / SUB;
+  constexpr uint64_t span = uint64_t(1) << kSubLog;
+  constexpr uint64_t bps = span / sublen;
+  static_assert(span % sublen == 0, "sub-range must divide the span");
+  const uint64_t win = bb / bps, off = bb % bps;
+  return win * span * SUB + off * sublen + (t / sublen) * span + (t % sublen);
+}
+
 // a*x + b*y, written in the reference's operand order (m00*in0 + m01*in1).
 template <typename T, typename V>
 __device__ __forceinline__ V cmad2(const Cx<T>& a, const V& x, const Cx<T>& b, const V& y)
@@ -94,18 +123,20 @@ __device__ __forceinline__ V cmul(const Cx<T>& a, const V& x)
 //   multi-controlled: K=1+#c,             mask0=ctrl,    mask1=ctrl|1<<t
 //   swap-like:        K=2 ins={p1,p2},    mask0=1<<p1,   mask1=1<<p2
 // MOVE=true exchanges the two amplitudes (X-type matrices: a pure, bit-exact move).
-template <typename T, int K, int P, bool MOVE>
-__global__ __launch_bounds__(kBlock) void k_pair(typename Vec<T>::t* __restrict__ st, uint64_t nitems,
-                                                 Ins<K> ins, uint64_t mask0, uint64_t mask1, Mat2<T> m)
+// TILED=true: XCD-contiguous, split-block tiling (nitems a power of two >= kTiledMin, grid
+// exactly nitems / (B*P)); TILED=false: plain contiguous blocks with bounds checks.
+template <typename T, int K, bool MOVE, bool TILED, int B, int P>
+__global__ __launch_bounds__(B) void k_pair(typename Vec<T>::t* __restrict__ st, uint64_t nitems, Ins<K> ins,
+                                            uint64_t mask0, uint64_t mask1, Mat2<T> m)
 {
   using V = typename Vec<T>::t;
-  const uint64_t k0 = (uint64_t)blockIdx.x * (kBlock * P) + threadIdx.x;
   V a[P], b[P];
   uint64_t base[P];
-  if (k0 + (uint64_t)(P - 1) * kBlock < nitems) {
+  if (TILED) {
+    const uint64_t bb = xcd_block(blockIdx.x, gridDim.x);
 #pragma unroll
     for (int j = 0; j < P; ++j) {
-      base[j] = item_base(k0 + (uint64_t)j * kBlock, ins);
+      base[j] = item_base(tiled_item<B, P, kSub>(bb, j * B + threadIdx.x), ins);
       a[j] = ldg(&st[base[j] | mask0]);
       b[j] = ldg(&st[base[j] | mask1]);
     }
@@ -121,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void k_pair(typename Vec<T>::t* __restrict_
     }
   } else {
     for (int j = 0; j < P; ++j) {
-      const uint64_t k = k0 + (uint64_t)j * kBlock;
+      const uint64_t k = (uint64_t)blockIdx.x * (B * P) + (uint64_t)j * B + threadIdx.x;
       if (k >= nitems) break;
       const uint64_t bb = item_base(k, ins);
       const V x = st[bb | mask0], y = st[bb | mask1];
@@ -136,29 +167,102 @@ __global__ __launch_bounds__(kBlock) void k_pair(typename Vec<T>::t* __restrict_
   }
 }
 
+// 2x2 update for a target below the wavefront width (pos < 6): lanes hold consecutive
+// amplitudes (fully coalesced 1 KiB accesses per wavefront instruction) and each lane
+// fetches its partner from lane ^ 2^pos.
+//   KC > 0: control positions (all >= 6) are inserted as ones, so only the controlled
+//           sub-state (N >> KC amplitudes) is read and written;
+//   COND:   full pass over the state, the update applied where (i & ctrl) == ctrl (for
+//           controls below 6, whose skipped amplitudes share cache lines with updated ones).
+// MOVE exchanges partners exactly (X-type matrices).  Requires (N >> KC) a multiple of
+// B*P and >= kTiledMin (grid a multiple of 8).
+template <typename T, int KC, bool COND, bool MOVE, int B, int P>
+__global__ __launch_bounds__(B) void k_pair_lane(typename Vec<T>::t* __restrict__ st, unsigned pos, Ins<KC> cins,
+                                                 uint64_t ctrl, Mat2<T> m)
+{
+  using V = typename Vec<T>::t;
+  const uint64_t r0 = xcd_block(blockIdx.x, gridDim.x) * (uint64_t)(B * P) + threadIdx.x;
+  V v[P];
+  uint64_t idx[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    const uint64_t r = r0 + (uint64_t)j * B;
+    idx[j] = KC > 0 ? (item_base(r, cins) | ctrl) : r;
+    v[j] = ldg(&st[idx[j]]);
+  }
+  const int msk = 1 << pos;
+  const bool upper = (threadIdx.x >> pos) & 1;
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    V p;
+    p.x = __shfl_xor(v[j].x, msk);
+    p.y = __shfl_xor(v[j].y, msk);
+    V o;
+    if (MOVE) o = p;
+    else o = upper ? cmad2(m.m10, p, m.m11, v[j]) : cmad2(m.m00, v[j], m.m01, p);
+    if (COND && (idx[j] & ctrl) != ctrl) o = v[j];
+    stg(o, &st[idx[j]]);
+  }
+}
+
+// Conditional pair update over the target only (controls below 3 share 128-byte lines
+// with uncontrolled amplitudes): every pair is read and written back, the update applied
+// where (i0 & ctrl) == ctrl.  Tiled mapping as k_pair.
+template <typename T, bool MOVE, int B, int P>
+__global__ __launch_bounds__(B) void k_pair_cond(typename Vec<T>::t* __restrict__ st, Ins<1> ins, uint64_t ctrl,
+                                                 Mat2<T> m)
+{
+  using V = typename Vec<T>::t;
+  const uint64_t bb = xcd_block(blockIdx.x, gridDim.x);
+  const uint64_t hb = uint64_t(1) << ins.p[0];
+  V a[P], b[P];
+  uint64_t base[P];
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    base[j] = item_base(tiled_item<B, P, kSub>(bb, j * B + threadIdx.x), ins);
+    a[j] = ldg(&st[base[j]]);
+    b[j] = ldg(&st[base[j] | hb]);
+  }
+#pragma unroll
+  for (int j = 0; j < P; ++j) {
+    V x = a[j], y = b[j];
+    if ((base[j] & ctrl) == ctrl) {
+      if (MOVE) {
+        x = b[j];
+        y = a[j];
+      } else {
+        x = cmad2(m.m00, a[j], m.m01, b[j]);
+        y = cmad2(m.m10, a[j], m.m11, b[j]);
+      }
+    }
+    stg(x, &st[base[j]]);
+    stg(y, &st[base[j] | hb]);
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // Scale kernel: st[b|mask] *= f for every item (K=0: the whole state).  Serves diagonal
 // gates whose other entries are exactly 1 (Z, S, T, CZ, CPhase, ...) by touching only the
 // amplitudes that change.
-template <typename T, int K, int P>
-__global__ __launch_bounds__(kBlock) void k_scale(typename Vec<T>::t* __restrict__ st, uint64_t nitems,
-                                                  Ins<K> ins, uint64_t mask, Cx<T> f)
+template <typename T, int K, bool TILED, int B, int P>
+__global__ __launch_bounds__(B) void k_scale(typename Vec<T>::t* __restrict__ st, uint64_t nitems, Ins<K> ins,
+                                             uint64_t mask, Cx<T> f)
 {
   using V = typename Vec<T>::t;
-  const uint64_t k0 = (uint64_t)blockIdx.x * (kBlock * P) + threadIdx.x;
-  if (k0 + (uint64_t)(P - 1) * kBlock < nitems) {
+  if (TILED) {
+    const uint64_t bb = xcd_block(blockIdx.x, gridDim.x);
     V a[P];
     uint64_t idx[P];
 #pragma unroll
     for (int j = 0; j < P; ++j) {
-      idx[j] = item_base(k0 + (uint64_t)j * kBlock, ins) | mask;
+      idx[j] = item_base(tiled_item<B, P, kSub>(bb, j * B + threadIdx.x), ins) | mask;
       a[j] = ldg(&st[idx[j]]);
     }
 #pragma unroll
     for (int j = 0; j < P; ++j) stg(cmul(f, a[j]), &st[idx[j]]);
   } else {
     for (int j = 0; j < P; ++j) {
-      const uint64_t k = k0 + (uint64_t)j * kBlock;
+      const uint64_t k = (uint64_t)blockIdx.x * (B * P) + (uint64_t)j * B + threadIdx.x;
       if (k >= nitems) break;
       const uint64_t i = item_base(k, ins) | mask;
       st[i] = cmul(f, st[i]);
@@ -167,22 +271,23 @@ __global__ __launch_bounds__(kBlock) void k_scale(typename Vec<T>::t* __restrict
 }
 
 // Elementwise 2-qubit diagonal: st[i] *= d[2*b(p1) + b(p2)].
-template <typename T, int P>
-__global__ __launch_bounds__(kBlock) void k_diag2(typename Vec<T>::t* __restrict__ st, uint64_t N, unsigned p1,
-                                                  unsigned p2, Cx<T> d0, Cx<T> d1, Cx<T> d2, Cx<T> d3)
+template <typename T, bool TILED, int B, int P>
+__global__ __launch_bounds__(B) void k_diag2(typename Vec<T>::t* __restrict__ st, uint64_t N, unsigned p1, unsigned p2,
+                                             Cx<T> d0, Cx<T> d1, Cx<T> d2, Cx<T> d3)
 {
   using V = typename Vec<T>::t;
-  const uint64_t i0 = (uint64_t)blockIdx.x * (kBlock * P) + threadIdx.x;
+  const uint64_t bb = TILED ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   V a[P];
+  uint64_t idx[P];
 #pragma unroll
   for (int j = 0; j < P; ++j) {
-    const uint64_t i = i0 + (uint64_t)j * kBlock;
-    if (i < N) a[j] = ldg(&st[i]);
+    idx[j] = TILED ? tiled_item<B, P, kSub>(bb, j * B + threadIdx.x) : bb * (uint64_t)(B * P) + (uint64_t)j * B + threadIdx.x;
+    if (TILED || idx[j] < N) a[j] = ldg(&st[idx[j]]);
   }
 #pragma unroll
   for (int j = 0; j < P; ++j) {
-    const uint64_t i = i0 + (uint64_t)j * kBlock;
-    if (i < N) {
+    const uint64_t i = idx[j];
+    if (TILED || i < N) {
       const bool b1 = (i >> p1) & 1, b2 = (i >> p2) & 1;
       const Cx<T> d = b1 ? (b2 ? d3 : d2) : (b2 ? d1 : d0);
       stg(cmul(d, a[j]), &st[i]);
@@ -191,19 +296,21 @@ __global__ __launch_bounds__(kBlock) void k_diag2(typename Vec<T>::t* __restrict
 }
 
 // Dense 4x4 on (ph, pl): indices b, b|dl, b|dh, b|dh|dl; new[t] = sum_c m[t][c] old[c].
-template <typename T, int P>
-__global__ __launch_bounds__(kBlock) void k_quad(typename Vec<T>::t* __restrict__ st, uint64_t nitems,
-                                                 Ins<2> ins, uint64_t dl, uint64_t dh, Mat4<T> m)
+template <typename T, bool TILED, int B, int P>
+__global__ __launch_bounds__(B) void k_quad(typename Vec<T>::t* __restrict__ st, uint64_t nitems, Ins<2> ins,
+                                            uint64_t dl, uint64_t dh, Mat4<T> m)
 {
   using V = typename Vec<T>::t;
-  const uint64_t k0 = (uint64_t)blockIdx.x * (kBlock * P) + threadIdx.x;
+  const uint64_t bb = TILED ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   V x[P][4];
   uint64_t base[P];
+  bool ok[P];
 #pragma unroll
   for (int j = 0; j < P; ++j) {
-    const uint64_t k = k0 + (uint64_t)j * kBlock;
+    const uint64_t k = TILED ? tiled_item<B, P, kSub>(bb, j * B + threadIdx.x) : bb * (uint64_t)(B * P) + (uint64_t)j * B + threadIdx.x;
+    ok[j] = TILED || k < nitems;
     base[j] = item_base(k, ins);
-    if (k < nitems) {
+    if (ok[j]) {
       x[j][0] = ldg(&st[base[j]]);
       x[j][1] = ldg(&st[base[j] | dl]);
       x[j][2] = ldg(&st[base[j] | dh]);
@@ -212,8 +319,7 @@ __global__ __launch_bounds__(kBlock) void k_quad(typename Vec<T>::t* __restrict_
   }
 #pragma unroll
   for (int j = 0; j < P; ++j) {
-    const uint64_t k = k0 + (uint64_t)j * kBlock;
-    if (k >= nitems) continue;
+    if (!ok[j]) continue;
     const uint64_t off[4] = {0, dl, dh, dh | dl};
 #pragma unroll
     for (int t = 0; t < 4; ++t) {

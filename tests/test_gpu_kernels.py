@@ -307,3 +307,91 @@ def test_errors_are_reported_not_fatal():
     other = StateVector(5, "c128")
     with pytest.raises(IqsError, match="differ"):
         sv.overlap(other)
+
+
+# ---- large enough registers for the tiled (XCD / split-block) and lane-shuffle paths ----
+NT = 17
+
+
+@pytest.fixture(scope="module")
+def big_states():
+    return {dt: rand_state(NT, dt, 1234) for dt in ("c128", "c64")}
+
+
+@pytest.mark.parametrize("pos", [0, 1, 3, 5, 6, 7, 10, 11, 12, 13, 16])
+def test_tiled_1q_dense(dt, big_states, pos):
+    h = big_states[dt].copy()
+    m = haar_unitary(2, SplitMix64(900 + pos))
+    sv = on_gpu(h, dt)
+    sv.apply_1q(pos, m)
+    oracle.apply_1q(h, pos, m)
+    assert np.abs(sv.download() - h).max() < TOL[dt]
+
+
+@pytest.mark.parametrize("c,t", [(0, 1), (1, 0), (16, 0), (0, 16), (5, 12), (12, 5), (15, 16)])
+def test_tiled_controlled_and_cnot(dt, big_states, c, t):
+    h = big_states[dt].copy()
+    m = haar_unitary(2, SplitMix64(c * 31 + t))
+    sv = on_gpu(h, dt)
+    sv.apply_c1q(c, t, m)
+    oracle.apply_c1q(h, c, t, m)
+    assert np.abs(sv.download() - h).max() < TOL[dt]
+    X = np.array([[0, 1], [1, 0]], dtype=complex)
+    sv.apply_c1q(t, c, X)
+    oracle.apply_c1q(h, t, c, X)
+    got = sv.download()
+    assert np.abs(got - h).max() < TOL[dt]
+
+
+@pytest.mark.parametrize("p1,p2", [(0, 1), (0, 16), (7, 9), (15, 2)])
+def test_tiled_swap_diag_2q(dt, big_states, p1, p2):
+    h = big_states[dt].copy()
+    sv = on_gpu(h, dt)
+    sv.apply_swaplike(p1, p2, [[0, 1], [1, 0]])
+    oracle.apply_swaplike(h, min(p1, p2), max(p1, p2), np.array([[0, 1], [1, 0]]))
+    assert np.array_equal(sv.download(), h)
+    d = np.exp(1j * np.array([0.1, 0.9, -0.4, 1.7]))
+    sv.apply_diag2(p1, p2, d)
+    oracle.apply_diag(h, p1, p2, d)
+    sv.apply_1q(p2, np.diag([np.exp(-0.2j), np.exp(0.6j)]))
+    oracle.apply_1q(h, p2, np.diag([np.exp(-0.2j), np.exp(0.6j)]))
+    sv.apply_c1q(p1, p2, np.diag([1, np.exp(0.3j)]))
+    oracle.apply_c1q(h, p1, p2, np.diag([1, np.exp(0.3j)]))
+    m4 = haar_unitary(4, SplitMix64(p1 + 50 * p2))
+    sv.apply_2q(p1, p2, m4)
+    oracle.apply_2q(h, p1, p2, m4)
+    assert np.abs(sv.download() - h).max() < 4 * TOL[dt]
+
+
+def test_tiled_toffoli_and_scale(dt, big_states):
+    h = big_states[dt].copy()
+    sv = on_gpu(h, dt)
+    sv.apply_mc1q([0, 16], 8, [[0, 1], [1, 0]])
+    j = np.arange(1 << NT)
+    sel = ((j & 1) == 1) & (((j >> 16) & 1) == 1)
+    want = h.copy()
+    want[sel] = h[sel ^ (1 << 8) * 0 + 0][...] if False else h[np.where(sel, j ^ (1 << 8), j)][sel]
+    assert np.array_equal(sv.download(), want)
+    sv.scale(0.5 + 0.5j)
+    assert np.abs(sv.download() - (0.5 + 0.5j) * want.astype(complex)).max() < TOL[dt]
+
+
+@pytest.mark.parametrize("controls,t", [([7, 16], 2), ([4], 2), ([9, 11, 15], 0), ([2, 13], 6), ([3], 10)])
+def test_tiled_multi_controlled_dispatch(dt, big_states, controls, t):
+    h = big_states[dt].copy()
+    m = haar_unitary(2, SplitMix64(sum(controls) * 7 + t))
+    sv = on_gpu(h, dt)
+    sv.apply_mc1q(controls, t, m)
+    j = np.arange(1 << NT)
+    cmask = sum(1 << c for c in controls)
+    sel = ((j & cmask) == cmask) & (((j >> t) & 1) == 0)
+    a, b = h[sel].astype(complex), h[j[sel] | (1 << t)].astype(complex)
+    want = h.astype(complex).copy()
+    want[sel] = m[0, 0] * a + m[0, 1] * b
+    want[j[sel] | (1 << t)] = m[1, 0] * a + m[1, 1] * b
+    assert np.abs(sv.download() - want).max() < TOL[dt]
+    sv.upload(h)
+    sv.apply_mc1q(controls, t, [[0, 1], [1, 0]])
+    want = h.copy()
+    want[sel], want[j[sel] | (1 << t)] = h[j[sel] | (1 << t)], h[sel]
+    assert np.array_equal(sv.download(), want)

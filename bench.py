@@ -46,12 +46,37 @@ def apply_op(sv, op, G):
 
 
 def kernel_kind(op):
+    """Which kernel a brick-circuit gate launches (iqs_capi.hip dispatch)."""
     name = op[0]
     if name in ("ApplyHadamard", "ApplyRotationX", "ApplyRotationY"):
-        return "dense1q"
+        return "dense1q_pair" if op[1] >= 6 else "dense1q_lane"
     if name == "ApplyRotationZ":
         return "diag1q"
     return "cnot"
+
+
+def traffic_from_profile(n: int):
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, "profiles", "k_pair_dense1q_30q_pmc_traffic.md")
+    if n != 30 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def calibrate(n: int) -> int:
+    """Run one device-to-device copy of a known size (for PMC counter calibration)."""
+    from iqs_amd.engine import StateVector, sync
+    a, b = StateVector(n, "c128"), StateVector(n, "c128")
+    a.init_uniform(0.5)
+    sync()
+    b.copy_from(a)
+    sync()
+    print(f"calibration copy: {a.size * 16} bytes read and {a.size * 16} bytes written")
+    return 0
 
 
 def cpu_baseline(n: int, layer_ops, budget_s: float = 30.0):
@@ -90,11 +115,15 @@ def main() -> int:
     ap.add_argument("--sweep-reps", type=int, default=3)
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true", help="one 2^28-amplitude complex128 copy, for PMC scaling")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this engine keeps the whole register in one "
               f"GPU's HBM and runs on one GPU only (use --gpus 1).", file=sys.stderr)
         return 2
+
+    if args.calibrate:
+        return calibrate(28)
 
     import numpy as np
     import torch
@@ -144,7 +173,7 @@ def main() -> int:
     per_kind = {}
     for kind, a, b in ev:
         per_kind.setdefault(kind, []).append(a.elapsed_time(b) * 1e-3)
-    dense = per_kind["dense1q"]
+    dense = per_kind["dense1q_pair"]
     dense_avg = sum(dense) / len(dense)
     achieved = 2 * S / dense_avg / 1e9
 
@@ -206,11 +235,13 @@ def main() -> int:
                                f"per layer), seed 20240602, state resident in HBM",
                    "qubits": n, "depth": args.steps, "gates": ngates},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_pair<double,1> (dense 1q update: H, RX, RY)",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(n),
+                     "kernel": "k_pair<double,1,false,true,1024,2> (dense 1q update, target >= 6: H, RX, RY)",
                      "avg_launch_ms": 1e3 * dense_avg, "launches": len(dense)},
         "per_kind_avg_ms": {k: 1e3 * sum(v) / len(v) for k, v in per_kind.items()},
         "hbm_frac_1q": achieved / HBM_PEAK_GBS,
+        "hbm_frac_1q_all_targets": (2 * S / (sum(dense + per_kind.get("dense1q_lane", [])) /
+                                             len(dense + per_kind.get("dense1q_lane", []))) / 1e9) / HBM_PEAK_GBS,
         "z_readout_first4": readout[:4],
         "sweep": sweep,
         "cpu_baseline": cpu,

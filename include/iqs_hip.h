@@ -64,7 +64,7 @@ int iqsh_timer_start(void);
 int iqsh_timer_stop(double *ms);
 
 /* ---- lifetime and data movement -------------------------------------------------- */
-/* Allocate a register of n qubits.  Replaces the reference's _mm_malloc allocation
+/* Allocate a register of n qubits (0 <= n <= 40; n = 0 holds one amplitude).  Replaces the reference's _mm_malloc allocation
  * (src/qureg_init.cpp:149-188).  Contents are undefined until initialized. */
 int iqsh_create(unsigned num_qubits, int dtype, iqsh_handle *out);
 /* Wrap existing device memory of 2^n amplitudes (not owned, not freed by destroy).
@@ -126,6 +126,9 @@ int iqsh_permute(iqsh_handle h, const unsigned *src_pos);
 /* ---- reductions (FP64) --------------------------------------------------------------- */
 /* sum |psi_i|^2 over bit pos = 1 (GetProbability, src/qureg_measure.cpp:136-178). */
 int iqsh_prob1(iqsh_handle h, unsigned pos, double *out);
+/* max |psi_i|^2 over the half with bit pos == value (the scans of IsClassicalBit /
+ * GetClassicalValue, src/qureg_measure.cpp:19-81, 184-262). */
+int iqsh_maxnorm_half(iqsh_handle h, unsigned pos, int value, double *out);
 /* sum |psi_i|^2 (ComputeNorm squared, src/qureg_utils.cpp:236-256). */
 int iqsh_norm2(iqsh_handle h, double *out);
 /* <psi|P|psi> for the Pauli string P with X-or-Y positions xmask, Y-or-Z positions zmask

@@ -422,6 +422,17 @@ __global__ __launch_bounds__(kBlock) void k_red_norm(const typename Vec<T>::t* _
   block_sum_store<1>(v, partial);
 }
 
+// max |psi_i|^2 over items (one inserted bit | mask)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_red_maxnorm(const typename Vec<T>::t* __restrict__ st, uint64_t items,
+                                                        Ins<1> ins, uint64_t mask, double* partial)
+{
+  double m = -1.0;
+  for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < items; k += (uint64_t)gridDim.x * kBlock)
+    m = fmax(m, norm2(st[item_base(k, ins) | mask]));
+  block_max_store(m, partial);
+}
+
 // Pauli string expectation + norm in one pass (see iqsh_expval_pauli).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_red_pauli(const typename Vec<T>::t* __restrict__ st, uint64_t N,
@@ -771,7 +782,7 @@ int iqsh_create(unsigned num_qubits, int dtype, iqsh_handle* out)
 {
   if (!out) return fail(IQSH_ERR_ARG, "null output handle");
   *out = nullptr;
-  if (num_qubits < 1 || num_qubits > 40) return fail(IQSH_ERR_ARG, "num_qubits %u out of range [1, 40]", num_qubits);
+  if (num_qubits > 40) return fail(IQSH_ERR_ARG, "num_qubits %u out of range [0, 40]", num_qubits);
   if (dtype != IQSH_C64 && dtype != IQSH_C128) return fail(IQSH_ERR_ARG, "bad dtype %d", dtype);
   auto* h = new iqsh_state;
   h->n = num_qubits;
@@ -1239,6 +1250,30 @@ int iqsh_prob1(iqsh_handle h, unsigned pos, double* out)
   if (int rc = check_pos(h, pos)) return rc;
   if (!out) return fail(IQSH_ERR_ARG, "null output");
   return DISPATCH(h, do_norm_sub)(h, 1, pos, out);
+}
+
+namespace {
+template <typename T>
+int do_maxnorm_half(iqsh_handle h, unsigned pos, int value, double* out)
+{
+  RedWork* w;
+  if (int rc = red_ws(&w)) return rc;
+  Ins<1> ins;
+  ins.p[0] = pos;
+  const unsigned g = red_grid(h->size() / 2);
+  k_red_maxnorm<T><<<g, kBlock, 0, g_stream>>>((const typename Vec<T>::t*)h->d, h->size() / 2, ins,
+                                               value ? (uint64_t(1) << pos) : 0, w->partial);
+  CHECK_LAUNCH();
+  return red_finish(w, g, 1, 1, out);
+}
+}  // namespace
+
+int iqsh_maxnorm_half(iqsh_handle h, unsigned pos, int value, double* out)
+{
+  if (int rc = check_h(h)) return rc;
+  if (int rc = check_pos(h, pos)) return rc;
+  if (!out) return fail(IQSH_ERR_ARG, "null output");
+  return DISPATCH(h, do_maxnorm_half)(h, pos, value, out);
 }
 
 int iqsh_norm2(iqsh_handle h, double* out)

@@ -1,0 +1,73 @@
+/// @file rng_utils.hpp
+/// Host random numbers with the stream scheme of the reference (include/rng_utils.hpp,
+/// src/rng_utils.cpp:84-164): three std::mt19937 streams (pool, state, local) seeded with
+/// seed, seed + 1 + state_id and seed + 1 + num_states + pool_rank, uniform numbers from
+/// std::uniform_real_distribution<Type>(0, 1).  Used for Initialize("rand").
+#pragma once
+
+#include <cassert>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "mpi_env.hpp"
+
+namespace iqs {
+
+template <typename Type>
+class RandomNumberGenerator
+{
+ public:
+  RandomNumberGenerator() = default;
+  explicit RandomNumberGenerator(RandomNumberGenerator* src) { *this = *src; }
+
+  std::size_t GetSeed() const { return seed_; }
+  void SetSeedStreamPtrs(std::size_t seed)
+  {
+    seed_ = seed;
+    n_local_ = n_state_ = n_pool_ = 0;
+    const int num_states = mpi::Environment::GetNumStates();
+    pool_.seed(seed + 0);
+    state_.seed(seed + 1 + mpi::Environment::GetStateId());
+    local_.seed(seed + 1 + num_states + mpi::Environment::GetPoolRank());
+  }
+  void SkipAhead(std::size_t num_skip, std::string shared = "local")
+  {
+    std::vector<Type> sink(num_skip);
+    UniformRandomNumbers(sink.data(), num_skip, Type(0), Type(1), shared);
+  }
+  void UniformRandomNumbers(Type* out, std::size_t size, Type a = 0, Type b = 1, std::string shared = "local")
+  {
+    std::mt19937* g = select(size, shared);
+    for (std::size_t i = 0; i < size; ++i) out[i] = a + (b - a) * u_(*g);
+  }
+  void GaussianRandomNumbers(Type* out, std::size_t size, std::string shared = "local")
+  {
+    std::mt19937* g = select(size, shared);
+    for (std::size_t i = 0; i < size; ++i) out[i] = n_(*g);
+  }
+  template <typename IntType>
+  void RandomIntegersInRange(IntType* out, std::size_t size, IntType a, IntType b, std::string shared = "local")
+  {
+    std::mt19937* g = select(size, shared);
+    std::uniform_int_distribution<IntType> d(a, b - 1);
+    for (std::size_t i = 0; i < size; ++i) out[i] = d(*g);
+  }
+
+ private:
+  std::mt19937* select(std::size_t size, const std::string& s)
+  {
+    if (s == "local") { n_local_ += size; return &local_; }
+    if (s == "state") { n_state_ += size; return &state_; }
+    if (s == "pool") { n_pool_ += size; return &pool_; }
+    throw std::invalid_argument("RNG stream must be local, state or pool");
+  }
+  std::size_t seed_ = 0;
+  std::size_t n_local_ = 0, n_state_ = 0, n_pool_ = 0;
+  std::mt19937 local_, state_, pool_;
+  std::uniform_real_distribution<Type> u_{Type(0), Type(1)};
+  std::normal_distribution<Type> n_{Type(0), Type(1)};
+};
+
+}  // namespace iqs

@@ -54,6 +54,7 @@ def _sig(L):
         "iqsh_permute": [P, P],
         "iqsh_prob1": [P, U, DP],
         "iqsh_norm2": [P, DP],
+        "iqsh_maxnorm_half": [P, U, I, DP],
         "iqsh_expval_pauli": [P, U64, U64, U, DP, DP],
         "iqsh_overlap": [P, P, DP, DP],
         "iqsh_maxabsdiff": [P, P, D, D, DP],

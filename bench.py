@@ -106,6 +106,53 @@ def cpu_baseline(n: int, layer_ops, budget_s: float = 30.0):
                       f"(intel-qs src/, -O3 OpenMP), per-gate steady_clock"}
 
 
+def qaoa_measure(n: int, p: int, full: bool, torch):
+    """28-qubit QAOA MaxCut gradient: adjoint vs parameter-shift on the same GPU."""
+    import numpy as np
+    from iqs_amd.qaoa import default_problem
+
+    prob, g, b = default_problem(n, p, "c128")
+    prob.adjoint(g, b)  # warm-up (allocations)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prob.forward_state(g, b)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    cost, dg, db = prob.adjoint(g, b)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    which = None if full else [("gamma", 0), ("beta", 0)]
+    t0 = time.perf_counter()
+    ps, nevals = prob.parameter_shift(g, b, which)
+    torch.cuda.synchronize()
+    t_ps = time.perf_counter() - t0
+    total_evals = p * 2 * (len(prob.edges) + n)
+    t_ps_full = t_ps if full else t_ps / nevals * total_evals
+    diff = max(abs((dg if k == "gamma" else db)[l] - v) for (k, l), v in ps.items())
+    fd = prob.finite_difference(g, b, "gamma", p - 1)
+    p64, g64, b64 = default_problem(n, p, "c64")
+    p64.renorm_every = 2
+    p64.adjoint(g64, b64)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    c2, dg2, db2 = p64.adjoint(g64, b64)
+    torch.cuda.synchronize()
+    t_adj64 = time.perf_counter() - t0
+    return {
+        "qubits": n, "layers": p, "edges": len(prob.edges), "params": 2 * p, "dtype": "f64",
+        "cost": cost, "forward_s": t_fwd, "adjoint_grad_s": t_adj,
+        "param_shift_evals_total": total_evals, "param_shift_evals_measured": nevals,
+        "param_shift_s_measured": t_ps,
+        "param_shift_grad_s": t_ps_full, "param_shift_full_run": full,
+        "speedup_adjoint_vs_param_shift": t_ps_full / t_adj,
+        "max_abs_diff_adjoint_vs_param_shift": diff,
+        "central_fd_h1e-4_gamma_last": fd, "adjoint_gamma_last": dg[-1],
+        "c64": {"adjoint_grad_s": t_adj64, "cost_abs_err_vs_f64": abs(c2 - cost),
+                "grad_max_abs_err_vs_f64": float(max(np.abs(dg2 - dg).max(), np.abs(db2 - db).max()))},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +163,9 @@ def main() -> int:
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true", help="one 2^28-amplitude complex128 copy, for PMC scaling")
+    ap.add_argument("--no-qaoa", action="store_true")
+    ap.add_argument("--qaoa-full", action="store_true", help="run all parameter-shift evaluations (minutes)")
+    ap.add_argument("--qaoa-qubits", type=int, default=28)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this engine keeps the whole register in one "
@@ -216,6 +266,8 @@ def main() -> int:
                 "per_target_GBs": [round(nbytes / t / 1e9, 1) for _, t in rows],
             }
 
+    del sv
+    qaoa = None if args.no_qaoa else qaoa_measure(args.qaoa_qubits, 8, args.qaoa_full, torch)
     cpu = None if args.no_cpu else cpu_baseline(n, layers[0])
 
     out = {
@@ -244,6 +296,7 @@ def main() -> int:
                                              len(dense + per_kind.get("dense1q_lane", []))) / 1e9) / HBM_PEAK_GBS,
         "z_readout_first4": readout[:4],
         "sweep": sweep,
+        "qaoa_grad": qaoa,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

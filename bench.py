@@ -153,6 +153,76 @@ def qaoa_measure(n: int, p: int, full: bool, torch):
     }
 
 
+def fusion_measure(n: int, ngates: int, torch):
+    """Random Haar 1q/2q circuit (seed 20240603) fused into <= 5-qubit blocks through the
+    drop-in (complex64 and complex128), against the FP64 unfused run of the same gates."""
+    import ctypes as C
+
+    import intelqs_py as iqs
+    from iqs_amd import runner
+    from iqs_amd._lib import call
+    from iqs_amd.circuits import random_1q2q_circuit
+
+    def handle(reg):
+        return C.c_void_p(reg.DeviceHandle())
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t0
+
+    def compare(reg, ref):
+        """(1 - fidelity, max-abs error) of reg against ref (complex128), FP64 reductions."""
+        dre, dim, n2a, n2b, mx = (C.c_double() for _ in range(5))
+        a = handle(reg)
+        tmp = None
+        if isinstance(reg, iqs.QubitRegisterSP):
+            tmp = C.c_void_p()
+            call("iqsh_create", n, 1, C.byref(tmp))
+            call("iqsh_convert", tmp, a)
+            a = tmp
+        b = handle(ref)
+        call("iqsh_overlap", a, b, C.byref(dre), C.byref(dim))
+        call("iqsh_norm2", a, C.byref(n2a))
+        call("iqsh_norm2", b, C.byref(n2b))
+        call("iqsh_maxabsdiff", a, b, 1.0, 0.0, C.byref(mx))
+        if tmp is not None:
+            call("iqsh_destroy", tmp)
+        f = (dre.value ** 2 + dim.value ** 2) / (n2a.value * n2b.value)
+        return 1.0 - f, mx.value, n2a.value
+
+    c = random_1q2q_circuit(n, ngates, 20240603)
+    ref = iqs.QubitRegister(n, "base", 0, 0)
+    _, t_ref = timed(lambda: runner.run(c, ref))
+    out = {"qubits": n, "gates": ngates, "block_qubits": 5, "seed": 20240603,
+           "f64_unfused_s": t_ref, "f64_unfused_gates_per_s": ngates / t_ref}
+    for prec, cls in (("c128", iqs.QubitRegister), ("c64", iqs.QubitRegisterSP)):
+        c.precision = "sp" if prec == "c64" else "dp"
+        if prec == "c64":
+            reg = cls(n, "base", 0, 0)
+            _, t_unf = timed(lambda: runner.run(c, reg))
+            inf_u, mx_u, _ = compare(reg, ref)
+            del reg
+        reg = cls(n, "base", 0, 0)
+        reg.SetFusionBlockQubits(5)
+        reg.TurnOnFusion()
+        _, t_f = timed(lambda: runner.run(c, reg))
+        gates, blocks = reg.FusionStats()
+        inf, mx, n2 = compare(reg, ref)
+        S = (1 << n) * (16 if prec == "c128" else 8)
+        o = {"fused_s": t_f, "fused_gates_per_s": ngates / t_f, "blocks": blocks, "gates_fused": gates,
+             "block_GBs_avg": 2 * S * blocks / t_f / 1e9, "one_minus_fidelity": inf, "max_abs_err": mx,
+             "norm2": n2}
+        if prec == "c64":
+            o.update({"unfused_s": t_unf, "unfused_one_minus_fidelity": inf_u, "unfused_max_abs_err": mx_u})
+        out[prec] = o
+        del reg
+    c.precision = "dp"
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,6 +236,8 @@ def main() -> int:
     ap.add_argument("--no-qaoa", action="store_true")
     ap.add_argument("--qaoa-full", action="store_true", help="run all parameter-shift evaluations (minutes)")
     ap.add_argument("--qaoa-qubits", type=int, default=28)
+    ap.add_argument("--no-fusion", action="store_true")
+    ap.add_argument("--fusion-qubits", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this engine keeps the whole register in one "
@@ -268,6 +340,7 @@ def main() -> int:
 
     del sv
     qaoa = None if args.no_qaoa else qaoa_measure(args.qaoa_qubits, 8, args.qaoa_full, torch)
+    fusion = None if args.no_fusion else fusion_measure(args.fusion_qubits, 200, torch)
     cpu = None if args.no_cpu else cpu_baseline(n, layers[0])
 
     out = {
@@ -297,6 +370,7 @@ def main() -> int:
         "z_readout_first4": readout[:4],
         "sweep": sweep,
         "qaoa_grad": qaoa,
+        "fusion": fusion,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -78,6 +78,9 @@ int iqsh_upload(iqsh_handle h, const void *host, uint64_t offset, uint64_t count
 int iqsh_download(iqsh_handle h, void *host, uint64_t offset, uint64_t count);
 /* dst := src (copy constructor, src/qureg_init.cpp:364-379). */
 int iqsh_copy(iqsh_handle dst, iqsh_handle src);
+/* dst := src converted to dst's precision (same number of qubits); used to compare a
+ * complex64 register against a complex128 one.  No counterpart in the reference. */
+int iqsh_convert(iqsh_handle dst, iqsh_handle src);
 
 /* ---- initialization (Initialize("base"|"++++"), src/qureg_init.cpp:225-359) ------- */
 int iqsh_init_basis(iqsh_handle h, uint64_t data_index);
@@ -110,6 +113,8 @@ int iqsh_apply_2q(iqsh_handle h, unsigned ph, unsigned pl, const double m[32]);
  * 1 <= k <= 5.  Replaces the reference's fused-gate replay (src/qureg_fusion.cpp:56-94):
  * a run of gates is multiplied into one block on the host and applied in one pass. */
 int iqsh_apply_kq(iqsh_handle h, unsigned k, const unsigned *pos, const double *U);
+/* Select the kernel for 5-qubit blocks: matrix cores (1, default) or vector ALUs (0). */
+int iqsh_set_block_mfma(int enable);
 /* state *= f (AmplitudeWiseScalarMultiplication / Normalize, src/qureg_utils.cpp:162-201). */
 int iqsh_scale(iqsh_handle h, double re, double im);
 /* h += f * x (AmplitudeWiseSum, src/qureg_utils.cpp:208-231). */

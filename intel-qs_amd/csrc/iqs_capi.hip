@@ -30,6 +30,7 @@ namespace {
 
 thread_local char g_err[512] = "";
 hipStream_t g_stream = nullptr;
+bool g_block_mfma = true;  // 5-qubit blocks on the matrix cores (iqsh_set_block_mfma)
 
 int fail(int code, const char* fmt, ...)
 {
@@ -115,6 +116,13 @@ inline unsigned grid_for(uint64_t items, int per_block)
   return (unsigned)std::max<uint64_t>(g, 1);
 }
 
+// 2D grid of `nblocks` workgroups (x fastest, at most 65536 in x).
+inline dim3 grid2(uint64_t nblocks)
+{
+  const uint64_t gx = nblocks < 65536 ? (nblocks ? nblocks : 1) : 65536;
+  return dim3((unsigned)gx, (unsigned)((nblocks + gx - 1) / gx), 1);
+}
+
 inline unsigned red_grid(uint64_t items)
 {
   // Fixed function of the problem size (deterministic summation order).
@@ -162,10 +170,10 @@ int launch_pair_m(iqsh_handle h, const Ins<K>& ins, uint64_t mask0, uint64_t mas
   using V = typename Vec<T>::t;
   const uint64_t items = h->size() >> K;
   if (items >= kTiledMin)
-    k_pair<T, K, MOVE, true, kTB, kTP><<<(unsigned)(items / (kTB * kTP)), kTB, 0, g_stream>>>((V*)h->d, items, ins, mask0,
+    k_pair<T, K, MOVE, true, kTB, kTP><<<grid2(items / (kTB * kTP)), kTB, 0, g_stream>>>((V*)h->d, items, ins, mask0,
                                                                                                  mask1, mat2<T>(m));
   else
-    k_pair<T, K, MOVE, false, kSB, kSP><<<grid_for(items, kSB * kSP), kSB, 0, g_stream>>>((V*)h->d, items, ins, mask0,
+    k_pair<T, K, MOVE, false, kSB, kSP><<<grid2(grid_for(items, kSB * kSP)), kSB, 0, g_stream>>>((V*)h->d, items, ins, mask0,
                                                                                           mask1, mat2<T>(m));
   CHECK_LAUNCH();
   return IQSH_OK;
@@ -182,7 +190,7 @@ int launch_lane(iqsh_handle h, unsigned t, const Ins<KC>& cins, uint64_t ctrl, c
 {
   using V = typename Vec<T>::t;
   const uint64_t items = h->size() >> KC;
-  k_pair_lane<T, KC, COND, MOVE, kLB, kLP><<<(unsigned)(items / (kLB * kLP)), kLB, 0, g_stream>>>((V*)h->d, t, cins, ctrl,
+  k_pair_lane<T, KC, COND, MOVE, kLB, kLP><<<grid2(items / (kLB * kLP)), kLB, 0, g_stream>>>((V*)h->d, t, cins, ctrl,
                                                                                                   mat2<T>(m));
   CHECK_LAUNCH();
   return IQSH_OK;
@@ -204,9 +212,9 @@ int launch_pair_cond(iqsh_handle h, unsigned t, uint64_t ctrl, const double* m, 
   const uint64_t items = h->size() >> 1;
   const unsigned grid = (unsigned)(items / (kTB * kTP));
   if (move)
-    k_pair_cond<T, true, kTB, kTP><<<grid, kTB, 0, g_stream>>>((V*)h->d, ins, ctrl, mat2<T>(m));
+    k_pair_cond<T, true, kTB, kTP><<<grid2(grid), kTB, 0, g_stream>>>((V*)h->d, ins, ctrl, mat2<T>(m));
   else
-    k_pair_cond<T, false, kTB, kTP><<<grid, kTB, 0, g_stream>>>((V*)h->d, ins, ctrl, mat2<T>(m));
+    k_pair_cond<T, false, kTB, kTP><<<grid2(grid), kTB, 0, g_stream>>>((V*)h->d, ins, ctrl, mat2<T>(m));
   CHECK_LAUNCH();
   return IQSH_OK;
 }
@@ -217,10 +225,10 @@ int launch_scale(iqsh_handle h, const Ins<K>& ins, uint64_t mask, const double* 
   using V = typename Vec<T>::t;
   const uint64_t items = h->size() >> K;
   if (items >= kTiledMin)
-    k_scale<T, K, true, kTB, kTP><<<(unsigned)(items / (kTB * kTP)), kTB, 0, g_stream>>>((V*)h->d, items, ins, mask,
+    k_scale<T, K, true, kTB, kTP><<<grid2(items / (kTB * kTP)), kTB, 0, g_stream>>>((V*)h->d, items, ins, mask,
                                                                                           cx<T>(f));
   else
-    k_scale<T, K, false, kSB, kSP><<<grid_for(items, kSB * kSP), kSB, 0, g_stream>>>((V*)h->d, items, ins, mask, cx<T>(f));
+    k_scale<T, K, false, kSB, kSP><<<grid2(grid_for(items, kSB * kSP)), kSB, 0, g_stream>>>((V*)h->d, items, ins, mask, cx<T>(f));
   CHECK_LAUNCH();
   return IQSH_OK;
 }
@@ -231,9 +239,9 @@ int launch_diag2(iqsh_handle h, unsigned p1, unsigned p2, Cx<T> d0, Cx<T> d1, Cx
   using V = typename Vec<T>::t;
   const uint64_t N = h->size();
   if (N >= kTiledMin)
-    k_diag2<T, true, kTB, kTP><<<(unsigned)(N / (kTB * kTP)), kTB, 0, g_stream>>>((V*)h->d, N, p1, p2, d0, d1, d2, d3);
+    k_diag2<T, true, kTB, kTP><<<grid2(N / (kTB * kTP)), kTB, 0, g_stream>>>((V*)h->d, N, p1, p2, d0, d1, d2, d3);
   else
-    k_diag2<T, false, kSB, kSP><<<grid_for(N, kSB * kSP), kSB, 0, g_stream>>>((V*)h->d, N, p1, p2, d0, d1, d2, d3);
+    k_diag2<T, false, kSB, kSP><<<grid2(grid_for(N, kSB * kSP)), kSB, 0, g_stream>>>((V*)h->d, N, p1, p2, d0, d1, d2, d3);
   CHECK_LAUNCH();
   return IQSH_OK;
 }
@@ -741,6 +749,12 @@ int iqsh_set_stream(void* stream)
   return IQSH_OK;
 }
 
+int iqsh_set_block_mfma(int enable)
+{
+  g_block_mfma = enable != 0;
+  return IQSH_OK;
+}
+
 int iqsh_sync(void)
 {
   HIP_TRY(hipStreamSynchronize(g_stream));
@@ -869,6 +883,23 @@ int do_fill(iqsh_handle h, double re, double im)
   return IQSH_OK;
 }
 }  // namespace
+
+int iqsh_convert(iqsh_handle dst, iqsh_handle src)
+{
+  if (int rc = check_h(dst)) return rc;
+  if (int rc = check_h(src)) return rc;
+  if (dst->n != src->n) return fail(IQSH_ERR_ARG, "convert between registers of different size");
+  if (dst->dtype == src->dtype) return iqsh_copy(dst, src);
+  const unsigned grid = (unsigned)std::min<uint64_t>(grid_for(dst->size(), kBlock), 65536);
+  if (dst->dtype == IQSH_C128)
+    k_convert<double, float><<<grid, kBlock, 0, g_stream>>>((Vec<double>::t*)dst->d, (const Vec<float>::t*)src->d,
+                                                            dst->size());
+  else
+    k_convert<float, double><<<grid, kBlock, 0, g_stream>>>((Vec<float>::t*)dst->d, (const Vec<double>::t*)src->d,
+                                                            dst->size());
+  CHECK_LAUNCH();
+  return IQSH_OK;
+}
 
 int iqsh_init_uniform(iqsh_handle h, double re, double im)
 {
@@ -1024,10 +1055,10 @@ int do_2q(iqsh_handle h, unsigned ph, unsigned pl, const double* m)
   const unsigned ins[2] = {ph, pl};
   const uint64_t items = h->size() >> 2;
   if (items >= kTiledMin)
-    k_quad<T, true, kTB, 1><<<(unsigned)(items / kTB), kTB, 0, g_stream>>>((V*)h->d, items, sorted_ins<2>(ins),
+    k_quad<T, true, kTB, 1><<<grid2(items / kTB), kTB, 0, g_stream>>>((V*)h->d, items, sorted_ins<2>(ins),
                                                                            uint64_t(1) << pl, uint64_t(1) << ph, M);
   else
-    k_quad<T, false, kSB, 2><<<grid_for(items, kSB * 2), kSB, 0, g_stream>>>((V*)h->d, items, sorted_ins<2>(ins),
+    k_quad<T, false, kSB, 2><<<grid2(grid_for(items, kSB * 2)), kSB, 0, g_stream>>>((V*)h->d, items, sorted_ins<2>(ins),
                                                                             uint64_t(1) << pl, uint64_t(1) << ph, M);
   CHECK_LAUNCH();
   return IQSH_OK;
@@ -1074,8 +1105,19 @@ int do_block_k(iqsh_handle h, const unsigned* pos, const double* U)
   int slot;
   if (int rc = ring_put(buf.data(), buf.size() * sizeof(T), &dU, &slot)) return rc;
   const uint64_t groups = h->size() >> KQ;
-  constexpr int G = kBlock / D;
-  k_block<T, KQ><<<grid_for(groups, G), kBlock, 0, g_stream>>>((V*)h->d, groups, ins, (const V*)dU);
+  if (KQ == 5 && groups >= 16 && g_block_mfma) {
+    // Matrix-core path: tiles of 16 groups, kTilesPerWave tiles per wavefront.
+    constexpr int kTilesPerWave = 8;
+    const uint64_t tiles = groups / 16;
+    const uint64_t waves = (tiles + kTilesPerWave - 1) / kTilesPerWave;
+    Ins<5> ins5;
+    for (int j = 0; j < 5; ++j) ins5.p[j] = ins.p[j < KQ ? j : 0];
+    k_block5_mfma<T, kTilesPerWave><<<grid2((waves + 3) / 4), kBlock, 0, g_stream>>>((V*)h->d, tiles, ins5,
+                                                                                          (const V*)dU);
+  } else {
+    constexpr int G = kBlock / D;
+    k_block<T, KQ><<<grid2(grid_for(groups, G)), kBlock, 0, g_stream>>>((V*)h->d, groups, ins, (const V*)dU);
+  }
   CHECK_LAUNCH();
   return ring_done(slot);
 }

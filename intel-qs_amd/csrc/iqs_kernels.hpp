@@ -84,6 +84,12 @@ constexpr uint64_t kTiledMin = uint64_t(1) << 14;
 
 __device__ __forceinline__ uint64_t xcd_block(uint64_t b, uint64_t G) { return (b & 7) * (G >> 3) + (b >> 3); }
 
+// Kernels that launch one workgroup per tile use a 2D grid (a dispatch holds fewer than
+// 2^32 work-items per dimension); the linear block id runs x fastest, so the dispatch
+// order -- and the round-robin over XCDs -- is that of a 1D grid.
+__device__ __forceinline__ uint64_t block_id() { return blockIdx.x + (uint64_t)blockIdx.y * gridDim.x; }
+__device__ __forceinline__ uint64_t grid_blocks() { return (uint64_t)gridDim.x * gridDim.y; }
+
 // Item index of (block bb, slot t in [0, B*P)) under the split-block tiling.
 template <int B, int P, int SUB>
 __device__ __forceinline__ uint64_t tiled_item(uint64_t bb, unsigned t)
@@ -133,7 +139,7 @@ __global__ __launch_bounds__(B) void k_pair(typename Vec<T>::t* __restrict__ st,
   V a[P], b[P];
   uint64_t base[P];
   if (TILED) {
-    const uint64_t bb = xcd_block(blockIdx.x, gridDim.x);
+    const uint64_t bb = xcd_block(block_id(), grid_blocks());
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       base[j] = item_base(tiled_item<B, P, kSub>(bb, j * B + threadIdx.x), ins);
@@ -152,7 +158,7 @@ __global__ __launch_bounds__(B) void k_pair(typename Vec<T>::t* __restrict__ st,
     }
   } else {
     for (int j = 0; j < P; ++j) {
-      const uint64_t k = (uint64_t)blockIdx.x * (B * P) + (uint64_t)j * B + threadIdx.x;
+      const uint64_t k = block_id() * (B * P) + (uint64_t)j * B + threadIdx.x;
       if (k >= nitems) break;
       const uint64_t bb = item_base(k, ins);
       const V x = st[bb | mask0], y = st[bb | mask1];
@@ -181,7 +187,7 @@ __global__ __launch_bounds__(B) void k_pair_lane(typename Vec<T>::t* __restrict_
                                                  uint64_t ctrl, Mat2<T> m)
 {
   using V = typename Vec<T>::t;
-  const uint64_t r0 = xcd_block(blockIdx.x, gridDim.x) * (uint64_t)(B * P) + threadIdx.x;
+  const uint64_t r0 = xcd_block(block_id(), grid_blocks()) * (uint64_t)(B * P) + threadIdx.x;
   V v[P];
   uint64_t idx[P];
 #pragma unroll
@@ -213,7 +219,7 @@ __global__ __launch_bounds__(B) void k_pair_cond(typename Vec<T>::t* __restrict_
                                                  Mat2<T> m)
 {
   using V = typename Vec<T>::t;
-  const uint64_t bb = xcd_block(blockIdx.x, gridDim.x);
+  const uint64_t bb = xcd_block(block_id(), grid_blocks());
   const uint64_t hb = uint64_t(1) << ins.p[0];
   V a[P], b[P];
   uint64_t base[P];
@@ -250,7 +256,7 @@ __global__ __launch_bounds__(B) void k_scale(typename Vec<T>::t* __restrict__ st
 {
   using V = typename Vec<T>::t;
   if (TILED) {
-    const uint64_t bb = xcd_block(blockIdx.x, gridDim.x);
+    const uint64_t bb = xcd_block(block_id(), grid_blocks());
     V a[P];
     uint64_t idx[P];
 #pragma unroll
@@ -262,7 +268,7 @@ __global__ __launch_bounds__(B) void k_scale(typename Vec<T>::t* __restrict__ st
     for (int j = 0; j < P; ++j) stg(cmul(f, a[j]), &st[idx[j]]);
   } else {
     for (int j = 0; j < P; ++j) {
-      const uint64_t k = (uint64_t)blockIdx.x * (B * P) + (uint64_t)j * B + threadIdx.x;
+      const uint64_t k = block_id() * (B * P) + (uint64_t)j * B + threadIdx.x;
       if (k >= nitems) break;
       const uint64_t i = item_base(k, ins) | mask;
       st[i] = cmul(f, st[i]);
@@ -276,7 +282,7 @@ __global__ __launch_bounds__(B) void k_diag2(typename Vec<T>::t* __restrict__ st
                                              Cx<T> d0, Cx<T> d1, Cx<T> d2, Cx<T> d3)
 {
   using V = typename Vec<T>::t;
-  const uint64_t bb = TILED ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const uint64_t bb = TILED ? xcd_block(block_id(), grid_blocks()) : block_id();
   V a[P];
   uint64_t idx[P];
 #pragma unroll
@@ -301,7 +307,7 @@ __global__ __launch_bounds__(B) void k_quad(typename Vec<T>::t* __restrict__ st,
                                             uint64_t dl, uint64_t dh, Mat4<T> m)
 {
   using V = typename Vec<T>::t;
-  const uint64_t bb = TILED ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const uint64_t bb = TILED ? xcd_block(block_id(), grid_blocks()) : block_id();
   V x[P][4];
   uint64_t base[P];
   bool ok[P];
@@ -353,7 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_block(typename Vec<T>::t* __restrict
 #pragma unroll
   for (int j = 0; j < KQ; ++j)
     if ((r >> j) & 1) off |= uint64_t(1) << ins.p[j];
-  const uint64_t grp = (uint64_t)blockIdx.x * G + g;
+  const uint64_t grp = block_id() * G + g;
   uint64_t idx = 0;
   if (grp < ngroups) {
     idx = item_base(grp, ins) | off;
@@ -376,6 +382,95 @@ __global__ __launch_bounds__(kBlock) void k_block(typename Vec<T>::t* __restrict
 }
 
 // ---------------------------------------------------------------------------------------
+// Dense 5-qubit block on the matrix cores.  y = U x for every group of 32 amplitudes, in the
+// real form [yr; yi] = [[Ur, -Ui], [Ui, Ur]] [xr; xi] (a 64 x 64 real matrix).  One
+// wavefront takes a tile of 16 groups: B = the 64 x 16 real tile, D = A B as 4 row blocks
+// of 16, K = 64 in 16 steps of mfma 16x16x4 (f32 for complex64, f64 for complex128).
+// Operand layouts (checked on gfx950 for every row and column): A lane l <-> A[l%16][l/16],
+// B lane l <-> B[l/16][l%16]; accumulator element i of lane l <-> D[4*(l/16)+i][l%16] for
+// the f32 instruction but D[(l/16)+4*i][l%16] for the f64 one.  So lane l (c = l/16) reads
+// rows c + 4j (j = 0..7) of group l%16 and writes rows {4c+i, 16+4c+i} (f32) or
+// {c+4i, 16+c+4i} (f64).  A stays in registers for all tiles of the wavefront; a tile is
+// read completely before any of it is written.
+template <typename T> struct MfmaAcc;
+template <> struct MfmaAcc<float> { typedef float t __attribute__((ext_vector_type(4))); };
+template <> struct MfmaAcc<double> { typedef double t __attribute__((ext_vector_type(4))); };
+
+__device__ __forceinline__ MfmaAcc<float>::t mfma16x16x4(float a, float b, MfmaAcc<float>::t d)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, d, 0, 0, 0);
+}
+__device__ __forceinline__ MfmaAcc<double>::t mfma16x16x4(double a, double b, MfmaAcc<double>::t d)
+{
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, d, 0, 0, 0);
+}
+
+template <typename T, int TPW>
+__global__ __launch_bounds__(kBlock) void k_block5_mfma(typename Vec<T>::t* __restrict__ st, uint64_t ntiles, Ins<5> ins,
+                                                        const typename Vec<T>::t* __restrict__ U)
+{
+  using V = typename Vec<T>::t;
+  using Acc = typename MfmaAcc<T>::t;
+  const int lane = threadIdx.x & 63, c = lane >> 4, gl = lane & 15;
+  const uint64_t wave = block_id() * (kBlock / 64) + (threadIdx.x >> 6);
+  const uint64_t t0 = wave * TPW;
+  if (t0 >= ntiles) return;
+  // A operands: a[mb][ks] = A[16 mb + gl][4 ks + c].
+  T a[4][16];
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) {
+      const int R = 16 * mb + gl, Cc = 4 * ks + c;
+      const V u = U[(R & 31) * 32 + (Cc & 31)];
+      a[mb][ks] = (R < 32) ? ((Cc < 32) ? u.x : -u.y) : ((Cc < 32) ? u.y : u.x);
+    }
+  // Row offsets: row bit j sets position ins.p[j].
+  uint64_t offin[8], offout[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int rin = c + 4 * j;
+    const int i = j & 3;
+    const int rin4 = (sizeof(T) == 8) ? (c + 4 * i) : (4 * c + i);  // accumulator row of element i
+    const int rout = (j < 4) ? rin4 : (16 + rin4);
+    uint64_t oi = 0, oo = 0;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+      if ((rin >> b) & 1) oi |= uint64_t(1) << ins.p[b];
+      if ((rout >> b) & 1) oo |= uint64_t(1) << ins.p[b];
+    }
+    offin[j] = oi;
+    offout[j] = oo;
+  }
+  const uint64_t t1 = (t0 + TPW < ntiles) ? t0 + TPW : ntiles;
+  for (uint64_t t = t0; t < t1; ++t) {
+    const uint64_t base = item_base(t * 16 + gl, ins);
+    V x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = ldg(&st[base | offin[j]]);
+    Acc d[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) d[mb] = Acc{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) {
+      const T bv = (ks < 8) ? x[ks].x : x[ks - 8].y;
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) d[mb] = mfma16x16x4(a[mb][ks], bv, d[mb]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      V y0, y1;
+      y0.x = d[0][i];
+      y0.y = d[2][i];
+      y1.x = d[1][i];
+      y1.y = d[3][i];
+      stg(y0, &st[base | offout[i]]);
+      stg(y1, &st[base | offout[4 + i]]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
 // Fills.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fill(typename Vec<T>::t* __restrict__ st, uint64_t N, Cx<T> v)
@@ -386,6 +481,20 @@ __global__ __launch_bounds__(kBlock) void k_fill(typename Vec<T>::t* __restrict_
   a.y = v.i;
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (uint64_t)gridDim.x * kBlock)
     stg(a, &st[i]);
+}
+
+// dst := src in dst's precision
+template <typename TD, typename TS>
+__global__ __launch_bounds__(kBlock) void k_convert(typename Vec<TD>::t* __restrict__ dst,
+                                                    const typename Vec<TS>::t* __restrict__ src, uint64_t N)
+{
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (uint64_t)gridDim.x * kBlock) {
+    const auto a = src[i];
+    typename Vec<TD>::t b;
+    b.x = (TD)a.x;
+    b.y = (TD)a.y;
+    dst[i] = b;
+  }
 }
 
 // h += f * x

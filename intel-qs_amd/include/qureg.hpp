@@ -142,6 +142,8 @@ class QubitRegister
   void ApplyFusedGates();
   void SetFusionBlockQubits(unsigned k);
   unsigned FusionBlockQubits() const { return fuse_k_; }
+  /// (queued gates applied so far, blocks launched for them) since construction.
+  std::pair<std::size_t, std::size_t> FusionStats() const { return {fused_gates_, fused_blocks_}; }
 
   // ---- specialization ----
   /// Matrix structure (identity, diagonal, X-type) is always exploited exactly by the
@@ -239,6 +241,7 @@ class QubitRegister
   mutable bool host_valid_ = false;
   mutable bool host_dirty_ = false;
   unsigned fuse_k_ = 5;
+  std::size_t fused_gates_ = 0, fused_blocks_ = 0;
   std::shared_ptr<std::vector<Pending>> queue_;
   RandomNumberGenerator<BaseType>* rng_ptr_ = nullptr;
   static bool do_print_extra_info;

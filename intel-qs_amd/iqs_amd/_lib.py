@@ -37,6 +37,7 @@ def _sig(L):
         "iqsh_upload": [P, P, U64, U64],
         "iqsh_download": [P, P, U64, U64],
         "iqsh_copy": [P, P],
+        "iqsh_convert": [P, P],
         "iqsh_init_basis": [P, U64],
         "iqsh_init_uniform": [P, D, D],
         "iqsh_apply_1q": [P, U, P],
@@ -48,6 +49,7 @@ def _sig(L):
         "iqsh_apply_diag2": [P, U, U, P],
         "iqsh_apply_2q": [P, U, U, P],
         "iqsh_apply_kq": [P, U, P, P],
+        "iqsh_set_block_mfma": [I],
         "iqsh_scale": [P, D, D],
         "iqsh_axpy": [P, P, D, D],
         "iqsh_collapse": [P, U, I],

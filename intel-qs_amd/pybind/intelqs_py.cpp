@@ -145,6 +145,7 @@ void bind_register(py::module_& m, const char* name)
       .def("IsFusionEnabled", &Q::IsFusionEnabled)
       .def("ApplyFusedGates", &Q::ApplyFusedGates)
       .def("SetFusionBlockQubits", &Q::SetFusionBlockQubits)
+      .def("FusionStats", &Q::FusionStats)
       // random numbers
       .def("ResetRngPtr", &Q::ResetRngPtr)
       .def("SetRngPtr", &Q::SetRngPtr)

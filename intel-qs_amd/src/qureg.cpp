@@ -841,6 +841,7 @@ void QubitRegister<Type>::ApplyFusedGates()
   }
   std::vector<Pending> q;
   q.swap(*queue_);
+  fused_gates_ += q.size();
   fwindow.clear();
   if (host_dirty_) {
     ck(iqsh_upload(h_, external_ ? external_ : mirror_.data(), 0, global_size_));
@@ -863,6 +864,7 @@ void QubitRegister<Type>::ApplyFusedGates()
     if (k == 1) ck(iqsh_apply_1q(h_, S[0], md.data()));
     else if (k == 2) ck(iqsh_apply_2q(h_, S[1], S[0], md.data()));  // row = 2*b(S[1]) + b(S[0])
     else ck(iqsh_apply_kq(h_, k, S.data(), md.data()));
+    ++fused_blocks_;
     S.clear();
     B.clear();
     ngates = 0;

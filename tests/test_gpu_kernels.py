@@ -395,3 +395,32 @@ def test_tiled_multi_controlled_dispatch(dt, big_states, controls, t):
     want = h.copy()
     want[sel], want[j[sel] | (1 << t)] = h[j[sel] | (1 << t)], h[sel]
     assert np.array_equal(sv.download(), want)
+
+
+@pytest.mark.parametrize("positions", [[0, 1, 2, 3, 4], [4, 3, 2, 1, 0], [13, 2, 7, 0, 10], [9, 10, 11, 12, 13],
+                                       [1, 5, 6, 11, 3]])
+def test_block5_mfma_vs_valu_and_gather(dt, positions):
+    from iqs_amd._lib import call
+    n = 14
+    U = haar_unitary(32, SplitMix64(sum(p * (i + 3) for i, p in enumerate(positions))))
+    h = rand_state(n, dt, 41)
+    sv = on_gpu(h, dt)
+    sv.apply_kq(positions, U)  # matrix cores
+    got = sv.download()
+    sv.upload(h)
+    call("iqsh_set_block_mfma", 0)
+    try:
+        sv.apply_kq(positions, U)  # vector ALUs
+    finally:
+        call("iqsh_set_block_mfma", 1)
+    valu = sv.download()
+    a = h.astype(np.complex128)
+    want = a.copy()
+    j = np.arange(1 << n)
+    mask = sum(1 << p for p in positions)
+    bases = j[(j & mask) == 0]
+    idx = np.stack([bases | sum(((r >> b) & 1) << positions[b] for b in range(5)) for r in range(32)])
+    want[idx] = U @ a[idx]
+    t = 1e-12 if dt == "c128" else 3e-5
+    assert np.abs(got - want).max() < t
+    assert np.abs(valu - want).max() < t

@@ -412,7 +412,9 @@ __global__ __launch_bounds__(kBlock) void k_block5_mfma(typename Vec<T>::t* __re
   using V = typename Vec<T>::t;
   using Acc = typename MfmaAcc<T>::t;
   const int lane = threadIdx.x & 63, c = lane >> 4, gl = lane & 15;
-  const uint64_t wave = block_id() * (kBlock / 64) + (threadIdx.x >> 6);
+  const uint64_t G = grid_blocks();
+  const uint64_t bb = (G & 7) ? block_id() : xcd_block(block_id(), G);
+  const uint64_t wave = bb * (kBlock / 64) + (threadIdx.x >> 6);
   const uint64_t t0 = wave * TPW;
   if (t0 >= ntiles) return;
   // A operands: a[mb][ks] = A[16 mb + gl][4 ks + c].
@@ -443,11 +445,19 @@ __global__ __launch_bounds__(kBlock) void k_block5_mfma(typename Vec<T>::t* __re
     offout[j] = oo;
   }
   const uint64_t t1 = (t0 + TPW < ntiles) ? t0 + TPW : ntiles;
-  for (uint64_t t = t0; t < t1; ++t) {
-    const uint64_t base = item_base(t * 16 + gl, ins);
-    V x[8];
+  // Software pipeline: the next tile's loads are in flight while this tile is multiplied.
+  uint64_t base = item_base(t0 * 16 + gl, ins);
+  V x[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) x[j] = ldg(&st[base | offin[j]]);
+  for (int j = 0; j < 8; ++j) x[j] = ldg(&st[base | offin[j]]);
+  for (uint64_t t = t0; t < t1; ++t) {
+    const bool more = t + 1 < t1;
+    const uint64_t nbase = more ? item_base((t + 1) * 16 + gl, ins) : base;
+    V xn[8];
+    if (more) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) xn[j] = ldg(&st[nbase | offin[j]]);
+    }
     Acc d[4];
 #pragma unroll
     for (int mb = 0; mb < 4; ++mb) d[mb] = Acc{0, 0, 0, 0};
@@ -467,6 +477,11 @@ __global__ __launch_bounds__(kBlock) void k_block5_mfma(typename Vec<T>::t* __re
       stg(y0, &st[base | offout[i]]);
       stg(y1, &st[base | offout[4 + i]]);
     }
+    if (more) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) x[j] = xn[j];
+    }
+    base = nbase;
   }
 }
 

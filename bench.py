@@ -18,6 +18,7 @@ Also reported:
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import math
 import os
@@ -223,6 +224,57 @@ def fusion_measure(n: int, ngates: int, torch):
     return out
 
 
+def batch_measure(reps: int, torch):
+    """512 hardware-efficient-ansatz instances (n ~ U{10..22}, d ~ U{10..50}, seed 20240606):
+    forward + adjoint gradient of every angle in one persistent, graph-replayed launch.
+    complex64 (autotuned) timed; complex128 run of the same batch as the FP64 reference;
+    parameter-shift spot check (one angle per instance) of the complex128 gradient."""
+    import numpy as np
+    from iqs_amd._lib import call
+    from iqs_amd.batch import HeaBatch, random_batch
+
+    nq, dp, ang = random_batch()
+    out = {"instances": len(nq), "seed": 20240606, "qubits": [min(nq), max(nq)], "depths": [min(dp), max(dp)],
+           "params": int(2 * sum(n * d for n, d in zip(nq, dp)))}
+    b64 = HeaBatch(nq, dp, "c64")
+    b64.set_angles(ang)
+    t0 = time.perf_counter()
+    tune, best_ms, trials = b64.autotune(grad=True, reps=2)
+    out["autotune_s"] = time.perf_counter() - t0
+    out["tune"] = {"variant": tune[0], "workers_per_cu": tune[1], "order": tune[2],
+                   "tile": HeaBatch.variants("c64")[tune[0]]}
+    out["trials_ms"] = [[list(t), round(ms, 3)] for t, ms in trials]
+
+    def timed(fn, k):
+        fn()
+        call("iqsh_sync")
+        call("iqsh_timer_start")
+        for _ in range(k):
+            fn()
+        ms = C.c_double()
+        call("iqsh_timer_stop", C.byref(ms))
+        return ms.value / k
+
+    ms_grad = timed(lambda: b64.run(grad=True), reps)
+    ms_fwd = timed(lambda: b64.run(grad=False), reps)
+    c64, g64 = b64.results()
+    traffic = b64.traffic(grad=True)
+    out["c64"] = {"ms_forward_and_gradient": ms_grad, "ms_forward_only": ms_fwd,
+                  "instances_per_s": len(nq) / (ms_grad * 1e-3),
+                  "algorithmic_bytes": traffic, "algorithmic_GBs": traffic / (ms_grad * 1e-3) / 1e9}
+    b64.close()
+    b128 = HeaBatch(nq, dp, "c128")
+    c128, g128 = b128.evaluate(ang)
+    out["c128"] = {"ms_forward_and_gradient": timed(lambda: b128.run(grad=True), max(1, reps // 2))}
+    ps = b128.parameter_shift(ang, [0] * len(nq))
+    b128.close()
+    out["c128"]["grad_vs_param_shift_max_abs"] = float(np.max(np.abs(g128[np.cumsum([0] + [2 * n * d for n, d in zip(nq, dp)])[:-1]] - ps)))
+    out["c64"].update({"cost_max_abs_err_vs_f64": float(np.max(np.abs(c64 - c128))),
+                       "grad_max_abs_err_vs_f64": float(np.max(np.abs(g64 - g128))),
+                       "grad_rms_err_vs_f64": float(np.sqrt(np.mean((g64 - g128) ** 2)))})
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,6 +289,8 @@ def main() -> int:
     ap.add_argument("--qaoa-full", action="store_true", help="run all parameter-shift evaluations (minutes)")
     ap.add_argument("--qaoa-qubits", type=int, default=28)
     ap.add_argument("--no-fusion", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--fusion-qubits", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -341,6 +395,7 @@ def main() -> int:
     del sv
     qaoa = None if args.no_qaoa else qaoa_measure(args.qaoa_qubits, 8, args.qaoa_full, torch)
     fusion = None if args.no_fusion else fusion_measure(args.fusion_qubits, 200, torch)
+    batch = None if args.no_batch else batch_measure(args.batch_reps, torch)
     cpu = None if args.no_cpu else cpu_baseline(n, layers[0])
 
     out = {
@@ -371,6 +426,7 @@ def main() -> int:
         "sweep": sweep,
         "qaoa_grad": qaoa,
         "fusion": fusion,
+        "hea_batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -182,6 +182,83 @@ int iqsh_qaoa_phase_adjoint(iqsh_handle psi, iqsh_handle lam, unsigned ne, const
 int iqsh_adjoint_1q(iqsh_handle psi, iqsh_handle lam, unsigned pos, const double Uinv[8],
                     const double *G, double *grad);
 
+/* ---- batches of hardware-efficient-ansatz circuits ------------------------------------
+ * No counterpart in the reference; the nearest is its pool of independent states
+ * (src/mpi_env.cpp:239-404, one state per MPI rank group).  A batch holds `count`
+ * independent registers, instance b with n_b qubits (1 <= n_b <= 26) and d_b >= 1 layers.
+ * Every instance starts from |0...0> and layer l applies, on every qubit q,
+ * RotationY(theta[l][q]) then RotationZ(phi[l][q]) (src/qureg_apply1qubitgate.cpp:297-325:
+ * RY = [[c,-s],[s,c]], RZ = diag(e^{-i phi/2}, e^{+i phi/2}), c,s of theta/2), followed by
+ * CPauliZ(i, i+1) for i = 0..n_b-2.  The cost is <sum_q Z_q> in the final state.
+ * Parameters and gradients share one layout: instance after instance, then layer, qubit,
+ * and the pair (theta, phi) -- 2 * n_b * d_b doubles per instance.
+ * A run executes forward passes and, with IQSH_HEA_GRAD, the adjoint gradient
+ * d cost / d angle of every angle, for all instances in one persistent kernel launch
+ * (plus a fill, a matrix set-up and a reduction kernel), captured in a HIP graph on first
+ * use and replayed afterwards.  Amplitudes are complex64 or complex128 (dtype); costs,
+ * gradient sums and their reductions are FP64 in a fixed order (bitwise reproducible for
+ * a given tuning). */
+typedef struct iqsh_hea *iqsh_hea_handle;
+
+enum iqsh_hea_flags {
+  IQSH_HEA_GRAD = 1,       /* adjoint gradient in addition to the cost */
+  IQSH_HEA_KEEP_STATE = 2, /* keep every final state (forward-only runs) for iqsh_hea_state */
+  IQSH_HEA_NO_GRAPH = 4    /* launch the kernels directly instead of replaying the graph */
+};
+
+/* Tuning of a batch: `variant` picks the kernel's tile (tile qubits, amplitudes per thread,
+ * see iqsh_hea_variant_info); `workers_per_cu` persistent workgroups per compute unit
+ * (0 = as many as fit); `order` the task order: 0 = tiled instances first, single-tile
+ * instances last; 1 = single-tile first; 2 = single-tile instances interleaved among the
+ * steps of the tiled ones; 3 = as 0 with every tiled instance's steps aligned to end on
+ * the last round. */
+typedef struct {
+  int variant;
+  int workers_per_cu;
+  int order;
+} iqsh_hea_tune;
+
+typedef struct {
+  iqsh_hea_tune tune;
+  double ms; /* mean milliseconds per run */
+} iqsh_hea_trial;
+
+/* Number of kernel variants for a dtype, and the shape of one: instances with
+ * n <= tile_qubits run whole in one workgroup; larger ones (n <= 2 * tile_qubits) run as
+ * tile passes.  `for_tests` variants have small tiles and exist to exercise the tiled path
+ * on small registers; the autotuner skips them. */
+int iqsh_hea_variants(int dtype, int *count);
+int iqsh_hea_variant_info(int dtype, int variant, int *tile_qubits, int *amps_per_thread,
+                          int *threads, int *for_tests);
+/* Create a batch (allocates every state and work buffer); tune NULL = default variant 0,
+ * as many workers as fit, order 0. */
+int iqsh_hea_create(int dtype, int count, const int *nqubits, const int *depth,
+                    const iqsh_hea_tune *tune, iqsh_hea_handle *out);
+int iqsh_hea_destroy(iqsh_hea_handle h);
+int iqsh_hea_num_params(iqsh_hea_handle h, uint64_t *nparams);
+int iqsh_hea_set_tune(iqsh_hea_handle h, const iqsh_hea_tune *tune);
+int iqsh_hea_get_tune(iqsh_hea_handle h, iqsh_hea_tune *tune);
+/* Angles from host memory or from device memory (both nparams doubles). */
+int iqsh_hea_set_angles(iqsh_hea_handle h, const double *host_angles);
+int iqsh_hea_set_angles_device(iqsh_hea_handle h, const double *device_angles);
+/* Run the batch (asynchronous on the library stream). */
+int iqsh_hea_run(iqsh_hea_handle h, int flags);
+/* Costs (count doubles) and, after a gradient run, gradients (nparams doubles); NULL
+ * skips one.  The host form synchronizes; the device form copies on the stream. */
+int iqsh_hea_results(iqsh_hea_handle h, double *cost, double *grad);
+int iqsh_hea_results_device(iqsh_hea_handle h, double *cost_dev, double *grad_dev);
+/* Final state of instance `inst` (2^n amplitudes of the batch dtype) after a run with
+ * IQSH_HEA_KEEP_STATE. */
+int iqsh_hea_state(iqsh_hea_handle h, int inst, void *host);
+/* Algorithmic HBM bytes of one run (amplitude reads and writes of the tiled instances;
+ * single-tile instances stay on chip). */
+int iqsh_hea_traffic(iqsh_hea_handle h, int flags, double *bytes);
+/* Try every tuning (non-test variants that fit the batch x workers {1, 2} x orders
+ * {0, 1, 2, 3}), `reps` timed runs each after one warm-up, keep the fastest.  Trials are
+ * written to `trials` (up to max_trials; *ntrials = how many were run). */
+int iqsh_hea_autotune(iqsh_hea_handle h, int flags, int reps, iqsh_hea_trial *trials,
+                      int max_trials, int *ntrials, double *best_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,1111 @@
+// Batches of hardware-efficient-ansatz circuits: forward pass, cost <sum_q Z_q> and the
+// adjoint gradient of every angle, for many small registers in one persistent launch.
+//
+// Circuit of instance b (n qubits, d layers), from |0...0>:
+//   layer l:  U_q = RZ(phi[l][q]) RY(theta[l][q]) on every qubit q, then CZ(i, i+1), i < n-1.
+// Adjoint method: psi = final state, lam = O psi with O = sum_q Z_q (diagonal, o(x) =
+// n - 2 popcount(x)); walking the gates backwards, the rotation exp(-i a P/2) contributes
+// dC/da = Im <lam|P|psi> at the point just after it, then it is undone on both vectors.
+// With psi', lam' taken after RZ on the same qubit (other qubits' rotations commute):
+//   dC/dphi   = Im <lam'|Z_q|psi'>
+//   dC/dtheta = cos(phi) Im <lam'|Y_q|psi'> - sin(phi) Im <lam'|X_q|psi'>
+// since RZ(phi) Y RZ(phi)^dagger = cos(phi) Y - sin(phi) X.  The three sums come from the
+// same amplitude pairs, so each qubit costs one pass over its pairs.
+//
+// Work decomposition (M = tile qubits, R = amplitudes per thread, NT = 2^M / R threads):
+//  * n <= M: the whole instance is one task.  Its state lives in registers (padded to 2^M
+//    amplitudes; padding stays zero) and LDS; nothing touches HBM but the results.
+//  * n > M (n <= 2M): tile passes.  An A tile holds the low M qubits of one value of the
+//    high ones; a B tile holds the n-M high qubits plus the c = 2M-n lowest (so each row
+//    of the tile is a contiguous run of 2^c amplitudes).  Pass j gates the qubits local to
+//    its locality L_j (A for even j, B for odd j):
+//        R_{j-1} on L_j,  CZ ladder of layer j-1,  R_j on L_j
+//    so one pass over the state applies about one layer: layer l's rotations are split
+//    between passes l and l+1 (they commute), and CZ of layer l sits between them.
+//    Forward passes j = 0..d-1, a turnaround pass j = d (forward, cost, lam, backward),
+//    backward passes j = d-1..0 on psi and lam: 2d+1 steps of 2^(n-M) tile tasks.
+//  * Inside a task the registers cycle through phases: phase k holds a run of log2(R)
+//    consecutive local bits (runs overlap by one bit), gates on those bits are register-
+//    local, and the layout changes through one LDS transpose per phase.  A CZ pair (i,i+1)
+//    is applied in the phase where its upper gated qubit first appears, after the R_{j-1}
+//    of both and before their R_j (a CZ is diagonal: it only needs the global index).
+//  * Scheduling: a global queue of tasks in a host-chosen order; a workgroup takes the
+//    next task with one atomic, and a tile task of step s waits until all tiles of step
+//    s-1 of its instance have arrived.  Every task waits only on tasks earlier in the
+//    queue, which were taken by running workgroups, so the queue always drains; waits are
+//    bounded and set an abort flag instead of hanging.
+//  * Gradient sums are FP64: per gate a wave sum, per tile a fixed-order sum over waves,
+//    written as per-tile partials and reduced in tile order by an epilogue kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/iqs_hip.h"
+#include "iqs_internal.hpp"
+#include "iqs_kernels.hpp"
+
+using iqsi::fail;
+
+namespace {
+
+template <typename T> using V = typename iqsk::Vec<T>::t;
+
+constexpr uint32_t kGrad = IQSH_HEA_GRAD, kKeep = IQSH_HEA_KEEP_STATE;
+
+struct HeaInst {
+  uint32_t n, depth;
+  uint32_t whole;   // 1: n <= tile qubits, the instance is a single task
+  uint32_t ntl;     // tiled: log2(tiles) = n - tile qubits
+  uint64_t psi, lam;  // amplitude offsets of the two vectors in the arena
+  uint64_t par;       // first (theta, phi) pair of the instance
+  uint64_t part;      // tiled: gradient partials [depth][n][3][tiles]
+  uint64_t cpart;     // tiled: cost partials [tiles]
+  uint32_t arrive;    // tiled: arrival counter of step 0 (one counter per step)
+  uint32_t pad;
+};
+
+struct HeaTask {
+  uint32_t inst;
+  uint16_t step;
+  uint16_t tile;
+};
+
+template <typename T> struct HeaArgs {
+  const HeaInst* inst;
+  const HeaTask* task;
+  uint32_t ntask;
+  uint32_t flags;
+  uint32_t* ctl;  // [0] queue head, [1] abort flag, [2 + i] arrival counter i
+  V<T>* arena;
+  const V<T>* mats;  // 4 entries per pair: U = RZ(phi) RY(theta), row-major
+  const double* ang;
+  double* grad;
+  double* cost;
+  double* part;
+  double* cpart;
+  uint32_t spin_limit;
+};
+
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
+
+__device__ __forceinline__ uint32_t bit_range(int lo, int hi)
+{
+  if (hi <= lo) return 0u;
+  const uint32_t top = hi >= 32 ? 0xffffffffu : ((1u << hi) - 1u);
+  return top & ~((1u << lo) - 1u);
+}
+
+// Local index of register k of thread tid when the registers hold local bits [a, a+LR).
+template <int LR>
+__device__ __forceinline__ uint32_t lidx(uint32_t tid, int k, int a)
+{
+  return (tid & ((1u << a) - 1u)) | ((uint32_t)k << a) | ((tid >> a) << (a + LR));
+}
+
+// LDS slot of local index u: the low LR bits are XOR-ed with the next LR bits so that
+// every layout's 16 consecutive lanes hit distinct banks.
+template <int LR>
+__device__ __forceinline__ uint32_t phys(uint32_t u)
+{
+  return u ^ ((u >> LR) & ((1u << LR) - 1u));
+}
+
+// Pass geometry: gated local bits [la, lb), the tile's local<->global maps and the phases.
+template <int M, int LR>
+struct Geo {
+  int n, kind, la, lb, c;  // kind 0: whole instance, 1: A tile, 2: B tile
+  uint32_t t;
+  int ne0, K;
+
+  __device__ void init(int n_, int kind_, int la_, int lb_, int c_, uint32_t t_)
+  {
+    n = n_;
+    kind = kind_;
+    la = la_;
+    lb = lb_;
+    c = c_;
+    t = t_;
+    const int a0 = min(la, M - LR);
+    ne0 = min(a0 + LR, lb);
+    K = 1 + (lb > ne0 ? (lb - ne0 + LR - 2) / (LR - 1) : 0);
+  }
+  __device__ int ne(int k) const { return min(ne0 + k * (LR - 1), lb); }
+  __device__ int nb(int k) const { return k ? ne(k - 1) : la; }
+  __device__ int a(int k) const { return min(k ? nb(k) - 1 : la, M - LR); }
+  __device__ int gq(int lbit) const { return (kind == 2 && lbit >= c) ? lbit - c + M : lbit; }
+  __device__ uint32_t gidx(uint32_t u) const
+  {
+    if (kind == 1) return u | (t << M);
+    if (kind == 2) return (u & ((1u << c) - 1u)) | (t << c) | ((u >> c) << M);
+    return u;
+  }
+  // CZ pairs (bit i = CZ(i, i+1)) applied in phase k.
+  __device__ uint32_t czmask(int k) const
+  {
+    const int g_ne = gq(ne(k) - 1) + 1;
+    uint32_t m = bit_range(k == 0 ? 0 : gq(nb(k)) - 1, g_ne - 1);
+    if (k == K - 1) m |= bit_range(gq(lb - 1), n - 1);
+    return m & bit_range(0, n - 1);
+  }
+};
+
+template <typename C>
+__device__ __forceinline__ void u_apply(const C* u, C& a, C& b)
+{
+  C r0, r1;
+  r0.x = (u[0].x * a.x - u[0].y * a.y) + (u[1].x * b.x - u[1].y * b.y);
+  r0.y = (u[0].x * a.y + u[0].y * a.x) + (u[1].x * b.y + u[1].y * b.x);
+  r1.x = (u[2].x * a.x - u[2].y * a.y) + (u[3].x * b.x - u[3].y * b.y);
+  r1.y = (u[2].x * a.y + u[2].y * a.x) + (u[3].x * b.y + u[3].y * b.x);
+  a = r0;
+  b = r1;
+}
+
+// U^dagger: rows (conj u00, conj u10), (conj u01, conj u11).
+template <typename C>
+__device__ __forceinline__ void ud_apply(const C* u, C& a, C& b)
+{
+  C r0, r1;
+  r0.x = (u[0].x * a.x + u[0].y * a.y) + (u[2].x * b.x + u[2].y * b.y);
+  r0.y = (u[0].x * a.y - u[0].y * a.x) + (u[2].x * b.y - u[2].y * b.x);
+  r1.x = (u[1].x * a.x + u[1].y * a.y) + (u[3].x * b.x + u[3].y * b.y);
+  r1.y = (u[1].x * a.y - u[1].y * a.x) + (u[3].x * b.y - u[3].y * b.x);
+  a = r0;
+  b = r1;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+template <typename T, int M, int R>
+struct Tile {
+  static constexpr int LR = ilog2(R);
+  static constexpr int NT = (1 << M) / R;
+  static constexpr int NW = NT / 64;
+  static constexpr int SLOTS = 6 * M;  // two layers x up to M gated bits x 3 sums
+  static_assert(R >= 4 && (R & (R - 1)) == 0, "R must be a power of two >= 4");
+  static_assert(NT >= 64 && NT <= 1024, "workgroup of 1..16 wavefronts");
+  using G = Geo<M, LR>;
+  using C = V<T>;
+
+  static __device__ void relayout(C* v, C* lds, int from, int to)
+  {
+    const uint32_t tid = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < R; ++k) lds[phys<LR>(lidx<LR>(tid, k, from))] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < R; ++k) v[k] = lds[phys<LR>(lidx<LR>(tid, k, to))];
+    __syncthreads();
+  }
+
+  // Gates U_q (forward) on the register bits in rmask; `row` = matrices of qubit 0 of a layer.
+  static __device__ void gates_fwd(C* x, uint32_t rmask, const C* row, int a, const G& g)
+  {
+#pragma unroll
+    for (int j = 0; j < LR; ++j) {
+      if ((rmask >> j) & 1u) {
+        const C* u = row + 4 * g.gq(a + j);
+        const C U[4] = {u[0], u[1], u[2], u[3]};
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+          if (!((k >> j) & 1)) u_apply(U, x[k], x[k | (1 << j)]);
+      }
+    }
+  }
+
+  static __device__ void cz_apply(C* x, uint32_t czm, int a, const G& g)
+  {
+    if (!czm) return;
+    const uint32_t tid = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const uint32_t xg = g.gidx(lidx<LR>(tid, k, a));
+      if (__popc(xg & (xg >> 1) & czm) & 1) x[k] = -x[k];
+    }
+  }
+
+  // Backward through U_q on the register bits in rmask, for psi (x) and lam (y): the three
+  // sums of each qubit are wave-reduced into redw[slot0 + 3 (lbit - la) + {0,1,2}].
+  static __device__ void gates_bwd(C* x, C* y, uint32_t rmask, const C* row, int a, const G& g, double* redw,
+                                   int slot0)
+  {
+    const bool lane0 = (threadIdx.x & 63) == 0;
+#pragma unroll
+    for (int j = 0; j < LR; ++j) {
+      if ((rmask >> j) & 1u) {
+        const C* u = row + 4 * g.gq(a + j);
+        const C U[4] = {u[0], u[1], u[2], u[3]};
+        double sx = 0, sy = 0, sz = 0;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          if ((k >> j) & 1) continue;
+          const int k1 = k | (1 << j);
+          const double p0r = x[k].x, p0i = x[k].y, p1r = x[k1].x, p1i = x[k1].y;
+          const double l0r = y[k].x, l0i = y[k].y, l1r = y[k1].x, l1i = y[k1].y;
+          // conj(l0) p1, conj(l1) p0, Im conj(l0) p0, Im conj(l1) p1
+          const double c01r = l0r * p1r + l0i * p1i, c01i = l0r * p1i - l0i * p1r;
+          const double c10r = l1r * p0r + l1i * p0i, c10i = l1r * p0i - l1i * p0r;
+          const double c00i = l0r * p0i - l0i * p0r, c11i = l1r * p1i - l1i * p1r;
+          sx += c01i + c10i;
+          sy += c10r - c01r;
+          sz += c00i - c11i;
+          ud_apply(U, x[k], x[k1]);
+          ud_apply(U, y[k], y[k1]);
+        }
+        sx = wave_sum(sx);
+        sy = wave_sum(sy);
+        sz = wave_sum(sz);
+        if (lane0) {
+          double* r = redw + slot0 + 3 * (a + j - g.la);
+          r[0] = sx;
+          r[1] = sy;
+          r[2] = sz;
+        }
+      }
+    }
+  }
+
+  // R_pre (layer row_pre), CZ (if cz), R_post (layer row_post) on the gated bits, phases
+  // 0..K-1.  A negative row skips that layer.
+  static __device__ void sweep_fwd(C* x, C* lds, int& cur, const G& g, const C* mats, int64_t row_pre,
+                                   int64_t row_post, bool cz)
+  {
+    for (int k = 0; k < g.K; ++k) {
+      const int nb = g.nb(k), ne = g.ne(k), a = g.a(k);
+      if (cur != a) {
+        relayout(x, lds, cur, a);
+        cur = a;
+      }
+      if (row_pre >= 0) gates_fwd(x, bit_range(nb - a, ne - a), mats + 4 * row_pre, a, g);
+      if (cz) cz_apply(x, g.czmask(k), a, g);
+      if (row_post >= 0) {
+        const int lo = max(nb - 1, g.la), hi = (k == g.K - 1) ? g.lb : ne - 1;
+        gates_fwd(x, bit_range(lo - a, hi - a), mats + 4 * row_post, a, g);
+      }
+    }
+  }
+
+  // Exact reverse of sweep_fwd on psi and lam, with the gradient sums: post-layer slots at
+  // 0, pre-layer slots at 3 (lb - la).
+  static __device__ void sweep_bwd(C* x, C* y, C* lds, int& cur, const G& g, const C* mats, int64_t row_pre,
+                                   int64_t row_post, bool cz, double* redw)
+  {
+    const int g3 = 3 * (g.lb - g.la);
+    for (int k = g.K - 1; k >= 0; --k) {
+      const int nb = g.nb(k), ne = g.ne(k), a = g.a(k);
+      if (cur != a) {
+        relayout(x, lds, cur, a);
+        relayout(y, lds, cur, a);
+        cur = a;
+      }
+      if (row_post >= 0) {
+        const int lo = max(nb - 1, g.la), hi = (k == g.K - 1) ? g.lb : ne - 1;
+        gates_bwd(x, y, bit_range(lo - a, hi - a), mats + 4 * row_post, a, g, redw, 0);
+      }
+      if (cz) {
+        const uint32_t m = g.czmask(k);
+        cz_apply(x, m, a, g);
+        cz_apply(y, m, a, g);
+      }
+      if (row_pre >= 0) gates_bwd(x, y, bit_range(nb - a, ne - a), mats + 4 * row_pre, a, g, redw, g3);
+    }
+  }
+
+  // Cost sum o(x)|psi_x|^2 of this workgroup's amplitudes (FP64, fixed order) and, if
+  // lam_out, lam = o psi.  Returns the total on thread 0.
+  static __device__ double cost_and_lam(C* x, C* y, bool lam_out, int cur, const G& g, double* redc)
+  {
+    const uint32_t tid = threadIdx.x;
+    double cs = 0;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const uint32_t xg = g.gidx(lidx<LR>(tid, k, cur));
+      const int o = g.n - 2 * __popc(xg);
+      const double re = x[k].x, im = x[k].y;
+      cs += (double)o * (re * re + im * im);
+      if (lam_out) y[k] = x[k] * (T)o;
+    }
+    cs = wave_sum(cs);
+    if ((tid & 63) == 0) redc[tid >> 6] = cs;
+    __syncthreads();
+    double tot = 0;
+    if (tid == 0)
+      for (int w = 0; w < NW; ++w) tot += redc[w];
+    return tot;
+  }
+
+  static __device__ void run_whole(const HeaArgs<T>& A, const HeaInst& I, uint32_t b, C* lds, double* red,
+                                   double* redc)
+  {
+    const uint32_t tid = threadIdx.x;
+    const int n = I.n, D = I.depth;
+    const bool grad = A.flags & kGrad;
+    G g;
+    g.init(n, 0, 0, n, 0, 0);
+    C x[R], y[R];
+    int cur = g.a(0);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      x[k] = C{0, 0};
+      if (lidx<LR>(tid, k, cur) == 0) x[k].x = 1;
+    }
+    for (int l = 0; l < D; ++l) sweep_fwd(x, lds, cur, g, A.mats, (int64_t)(I.par + (uint64_t)l * n), -1, true);
+    const double cost = cost_and_lam(x, y, grad, cur, g, redc);
+    if (tid == 0) A.cost[b] = cost;
+    if (grad) {
+      double* redw = red + (tid >> 6) * SLOTS;
+      for (int l = D - 1; l >= 0; --l) {
+        const uint64_t row = I.par + (uint64_t)l * n;
+        sweep_bwd(x, y, lds, cur, g, A.mats, (int64_t)row, -1, true, redw);
+        __syncthreads();
+        for (int q = tid; q < n; q += NT) {
+          double s[3] = {0, 0, 0};
+          for (int w = 0; w < NW; ++w)
+            for (int kk = 0; kk < 3; ++kk) s[kk] += red[w * SLOTS + 3 * n + 3 * q + kk];
+          const double ph = A.ang[2 * (row + q) + 1];
+          A.grad[2 * (row + q)] = cos(ph) * s[1] - sin(ph) * s[0];
+          A.grad[2 * (row + q) + 1] = s[2];
+        }
+        __syncthreads();
+      }
+    }
+    if (A.flags & kKeep) {
+      C* psi = A.arena + I.psi;
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const uint32_t u = lidx<LR>(tid, k, cur);
+        if (u < (1u << n)) psi[u] = x[k];
+      }
+    }
+  }
+
+  static __device__ void run_pass(const HeaArgs<T>& A, const HeaInst& I, int s, uint32_t t, C* lds, double* red,
+                                  double* redc)
+  {
+    const uint32_t tid = threadIdx.x;
+    const int n = I.n, D = I.depth;
+    const bool grad = A.flags & kGrad;
+    const int j = s <= D ? s : 2 * D - s;
+    const bool isB = j & 1;
+    const int c = 2 * M - n;
+    G g;
+    g.init(n, isB ? 2 : 1, isB ? c : 0, M, c, t);
+    const int64_t row_pre = j >= 1 ? (int64_t)(I.par + (uint64_t)(j - 1) * n) : -1;
+    const int64_t row_post = j < D ? (int64_t)(I.par + (uint64_t)j * n) : -1;
+    const bool cz = j >= 1;
+    constexpr int L = M - LR;  // load/store layout: consecutive threads, consecutive amplitudes
+    C* psi = A.arena + I.psi;
+    C* lam = A.arena + I.lam;
+    C x[R], y[R];
+    if (s == 0) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        x[k] = C{0, 0};
+        if (g.gidx(lidx<LR>(tid, k, L)) == 0) x[k].x = 1;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < R; ++k) x[k] = psi[g.gidx(lidx<LR>(tid, k, L))];
+    }
+    if (s > D) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) y[k] = lam[g.gidx(lidx<LR>(tid, k, L))];
+    }
+    int cur = L;
+    double* redw = red + (tid >> 6) * SLOTS;
+    if (s <= D) sweep_fwd(x, lds, cur, g, A.mats, row_pre, row_post, cz);
+    if (s == D) {
+      const double cs = cost_and_lam(x, y, grad, cur, g, redc);
+      if (tid == 0) A.cpart[I.cpart + t] = cs;
+    }
+    if (grad && s >= D) sweep_bwd(x, y, lds, cur, g, A.mats, row_pre, row_post, cz, redw);
+    const bool store_x = s < D || (s == D && (grad || (A.flags & kKeep))) || (s > D && s < 2 * D);
+    const bool store_y = grad && s >= D && s < 2 * D;
+    if ((store_x || store_y) && cur != L) {
+      relayout(x, lds, cur, L);
+      if (store_y) relayout(y, lds, cur, L);
+      cur = L;
+    }
+    if (store_x) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) psi[g.gidx(lidx<LR>(tid, k, L))] = x[k];
+    }
+    if (store_y) {
+#pragma unroll
+      for (int k = 0; k < R; ++k) lam[g.gidx(lidx<LR>(tid, k, L))] = y[k];
+    }
+    if (grad && s >= D) {
+      __syncthreads();
+      const int gg = g.lb - g.la;
+      for (int sl = tid; sl < 6 * gg; sl += NT) {
+        const int which = sl / (3 * gg);  // 0: layer j (post), 1: layer j-1 (pre)
+        const int layer = which ? j - 1 : j;
+        if (layer < 0 || layer >= D) continue;
+        double v = 0;
+        for (int w = 0; w < NW; ++w) v += red[w * SLOTS + sl];
+        const int r = sl - which * 3 * gg;
+        const int q = g.gq(g.la + r / 3);
+        A.part[I.part + ((((uint64_t)layer * n + q) * 3 + r % 3) << I.ntl) + t] = v;
+      }
+    }
+  }
+};
+
+template <typename T, int M, int R>
+__global__ __launch_bounds__((Tile<T, M, R>::NT)) void k_hea(HeaArgs<T> A)
+{
+  using Tl = Tile<T, M, R>;
+  __shared__ V<T> lds[1 << M];
+  __shared__ double red[Tl::NW * Tl::SLOTS];
+  __shared__ double redc[Tl::NW];
+  __shared__ uint32_t sh[2];
+  const uint32_t tid = threadIdx.x;
+  for (;;) {
+    if (tid == 0) {
+      uint32_t i = atomicAdd(A.ctl, 1u);
+      if (__hip_atomic_load(A.ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) i = A.ntask;
+      sh[0] = i;
+    }
+    __syncthreads();
+    const uint32_t i = sh[0];
+    if (i >= A.ntask) return;
+    const HeaTask tk = A.task[i];
+    const HeaInst I = A.inst[tk.inst];
+    if (I.whole) {
+      Tl::run_whole(A, I, tk.inst, lds, red, redc);
+    } else {
+      if (tid == 0) {
+        uint32_t ok = 1;
+        if (tk.step > 0) {
+          uint32_t* ctr = A.ctl + 2 + I.arrive + tk.step - 1;
+          const uint32_t need = 1u << I.ntl;
+          uint32_t spins = 0;
+          while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+            if (__hip_atomic_load(A.ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || ++spins > A.spin_limit) {
+              __hip_atomic_store(A.ctl + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              ok = 0;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+          }
+        }
+        sh[1] = ok;
+      }
+      __syncthreads();
+      if (sh[1]) {
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        Tl::run_pass(A, I, tk.step, tk.tile, lds, red, redc);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (tid == 0)
+          __hip_atomic_fetch_add(A.ctl + 2 + I.arrive + tk.step, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_hea_zero(uint32_t* p, uint32_t n)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// U = RZ(phi) RY(theta) per pair, computed in FP64 and rounded once to T.
+template <typename T>
+__global__ void k_hea_mats(const double* __restrict__ ang, V<T>* __restrict__ mats, uint64_t npairs)
+{
+  const uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  double s, c, sp, cp;
+  sincos(0.5 * ang[2 * p], &s, &c);
+  sincos(0.5 * ang[2 * p + 1], &sp, &cp);
+  mats[4 * p + 0] = V<T>{(T)(cp * c), (T)(-sp * c)};
+  mats[4 * p + 1] = V<T>{(T)(-cp * s), (T)(sp * s)};
+  mats[4 * p + 2] = V<T>{(T)(cp * s), (T)(sp * s)};
+  mats[4 * p + 3] = V<T>{(T)(cp * c), (T)(sp * c)};
+}
+
+// One block per (tiled instance, layer): thread q sums the tile partials of qubit q in
+// tile order and combines them into (d/dtheta, d/dphi); the costs are summed likewise.
+__global__ void k_hea_epi(const HeaInst* __restrict__ inst, const uint2* __restrict__ items,
+                          const double* __restrict__ ang, const double* __restrict__ part,
+                          const double* __restrict__ cpart, double* grad, double* cost, int with_grad)
+{
+  const uint2 it = items[blockIdx.x];
+  const HeaInst I = inst[it.x];
+  const uint32_t q = threadIdx.x;
+  const uint64_t T = uint64_t(1) << I.ntl;
+  if (with_grad && q < I.n) {
+    const uint64_t pi = I.par + (uint64_t)it.y * I.n + q;
+    const double* p = part + I.part + ((((uint64_t)it.y * I.n + q) * 3) << I.ntl);
+    double sx = 0, sy = 0, sz = 0;
+    for (uint64_t t = 0; t < T; ++t) {
+      sx += p[t];
+      sy += p[T + t];
+      sz += p[2 * T + t];
+    }
+    const double ph = ang[2 * pi + 1];
+    grad[2 * pi] = cos(ph) * sy - sin(ph) * sx;
+    grad[2 * pi + 1] = sz;
+  }
+  if (it.y == 0 && q == blockDim.x - 1) {
+    double v = 0;
+    for (uint64_t t = 0; t < T; ++t) v += cpart[I.cpart + t];
+    cost[it.x] = v;
+  }
+}
+
+// ---- variants ------------------------------------------------------------------------
+struct Variant {
+  int M, R, test;
+};
+constexpr Variant kVar64[] = {{13, 32, 0}, {13, 16, 0}, {12, 16, 0}, {8, 4, 1}};
+constexpr Variant kVar128[] = {{12, 16, 0}, {12, 8, 0}, {11, 8, 0}, {8, 4, 1}};
+constexpr int kNumVar = 4;
+
+const Variant& variant(int dtype, int v) { return dtype == IQSH_C128 ? kVar128[v] : kVar64[v]; }
+
+template <typename T, int M, int R>
+hipError_t launch_t(const HeaArgs<T>& a, int workers_per_cu, int cus, hipStream_t st)
+{
+  constexpr int NT = Tile<T, M, R>::NT;
+  int fit = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, k_hea<T, M, R>, NT, 0);
+  if (e != hipSuccess) return e;
+  fit = std::max(fit, 1);
+  const int w = workers_per_cu > 0 ? std::min(workers_per_cu, fit) : fit;
+  hipLaunchKernelGGL((k_hea<T, M, R>), dim3(std::max(1, w * cus)), dim3(NT), 0, st, a);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_var(int v, const HeaArgs<T>& a, int wpc, int cus, hipStream_t st);
+template <>
+hipError_t launch_var<float>(int v, const HeaArgs<float>& a, int wpc, int cus, hipStream_t st)
+{
+  switch (v) {
+    case 0: return launch_t<float, 13, 32>(a, wpc, cus, st);
+    case 1: return launch_t<float, 13, 16>(a, wpc, cus, st);
+    case 2: return launch_t<float, 12, 16>(a, wpc, cus, st);
+    default: return launch_t<float, 8, 4>(a, wpc, cus, st);
+  }
+}
+template <>
+hipError_t launch_var<double>(int v, const HeaArgs<double>& a, int wpc, int cus, hipStream_t st)
+{
+  switch (v) {
+    case 0: return launch_t<double, 12, 16>(a, wpc, cus, st);
+    case 1: return launch_t<double, 12, 8>(a, wpc, cus, st);
+    case 2: return launch_t<double, 11, 8>(a, wpc, cus, st);
+    default: return launch_t<double, 8, 4>(a, wpc, cus, st);
+  }
+}
+
+constexpr uint32_t kSpinLimit = 1u << 20;  // ~1 s of polling before a wait gives up
+constexpr int kMaxQubits = 26;
+constexpr int kMaxDepth = 10000;
+
+}  // namespace
+
+struct iqsh_hea {
+  int dtype = IQSH_C64;
+  int count = 0;
+  std::vector<int> n, d;
+  std::vector<uint64_t> par, soff;
+  uint64_t npairs = 0, arena_amps = 0;
+  iqsh_hea_tune tune{0, 0, 0};
+  int cus = 0;
+  hipStream_t cap = nullptr;  // private stream for graph capture
+
+  void* arena = nullptr;
+  void* mats = nullptr;
+  double* ang = nullptr;
+  double* grad = nullptr;
+  double* cost = nullptr;
+  double* part = nullptr;
+  size_t part_cap = 0;
+  double* cpart = nullptr;
+  size_t cpart_cap = 0;
+  HeaInst* inst = nullptr;
+  HeaTask* task = nullptr;
+  size_t task_cap = 0;
+  uint32_t* ctl = nullptr;
+  size_t ctl_cap = 0;
+  uint2* epi = nullptr;
+  size_t epi_cap = 0;
+
+  int built_mode = -1;  // flags (GRAD | KEEP) the schedule was built for
+  uint32_t ntask = 0, nepi = 0, nctl = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_mode = -1;
+  bool kept = false;
+
+  size_t esz() const { return dtype == IQSH_C128 ? 16 : 8; }
+};
+
+namespace {
+
+void drop_graph(iqsh_hea* h)
+{
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  h->gexec = nullptr;
+  h->graph = nullptr;
+  h->graph_mode = -1;
+}
+
+template <typename P>
+int grow(P*& p, size_t& cap, size_t need, const char* what)
+{
+  if (need <= cap && p) return IQSH_OK;
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  cap = 0;
+  const size_t cnt = std::max<size_t>(need, 1);
+  hipError_t e = hipMalloc((void**)&p, cnt * sizeof(P));
+  if (e != hipSuccess) return fail(IQSH_ERR_ALLOC, "hipMalloc(%zu bytes) for %s: %s", cnt * sizeof(P), what,
+                                   hipGetErrorString(e));
+  cap = cnt;
+  return IQSH_OK;
+}
+
+int steps_of(int d, bool grad) { return grad ? 2 * d + 1 : d + 1; }
+
+// Instance table, task queue and epilogue items for the current tuning and `mode`.
+int build(iqsh_hea* h, int mode)
+{
+  const Variant& v = variant(h->dtype, h->tune.variant);
+  const bool grad = mode & kGrad;
+  std::vector<HeaInst> I(h->count);
+  uint64_t part = 0, cpart = 0;
+  uint32_t arrive = 0;
+  std::vector<int> tiled, whole;
+  int smax = 0;
+  for (int b = 0; b < h->count; ++b) {
+    HeaInst& x = I[b];
+    std::memset(&x, 0, sizeof x);
+    x.n = h->n[b];
+    x.depth = h->d[b];
+    x.whole = h->n[b] <= v.M;
+    x.psi = h->soff[b];
+    x.lam = h->soff[b] + (uint64_t(1) << h->n[b]);
+    x.par = h->par[b];
+    if (x.whole) {
+      whole.push_back(b);
+      continue;
+    }
+    if (h->n[b] > 2 * v.M)
+      return fail(IQSH_ERR_NOTSUP, "instance %d: %d qubits exceed twice the tile (%d qubits)", b, h->n[b], v.M);
+    x.ntl = h->n[b] - v.M;
+    x.part = part;
+    part += (uint64_t)h->d[b] * h->n[b] * 3 << x.ntl;
+    x.cpart = cpart;
+    cpart += uint64_t(1) << x.ntl;
+    x.arrive = arrive;
+    arrive += steps_of(h->d[b], grad);
+    smax = std::max(smax, steps_of(h->d[b], grad));
+    tiled.push_back(b);
+  }
+  // Larger registers first within a round; costliest single-tile instances first.
+  auto by_size = [&](int a, int b) {
+    if (h->n[a] != h->n[b]) return h->n[a] > h->n[b];
+    if (h->d[a] != h->d[b]) return h->d[a] > h->d[b];
+    return a < b;
+  };
+  std::stable_sort(tiled.begin(), tiled.end(), by_size);
+  std::stable_sort(whole.begin(), whole.end(), by_size);
+
+  std::vector<HeaTask> tasks;
+  const int order = h->tune.order;
+  auto push_whole = [&](size_t lo, size_t hi) {
+    for (size_t i = lo; i < hi; ++i) tasks.push_back(HeaTask{(uint32_t)whole[i], 0, 0});
+  };
+  if (order == 1) push_whole(0, whole.size());
+  size_t wnext = 0;
+  for (int r = 0; r < smax; ++r) {
+    for (int b : tiled) {
+      const int S = steps_of(h->d[b], grad);
+      const int s = r - (order == 3 ? smax - S : 0);
+      if (s < 0 || s >= S) continue;
+      const uint32_t T = 1u << I[b].ntl;
+      for (uint32_t t = 0; t < T; ++t) tasks.push_back(HeaTask{(uint32_t)b, (uint16_t)s, (uint16_t)t});
+    }
+    if (order == 2) {
+      const size_t upto = whole.size() * (size_t)(r + 1) / (size_t)smax;
+      push_whole(wnext, upto);
+      wnext = upto;
+    }
+  }
+  if (order == 2) push_whole(wnext, whole.size());
+  if (order == 0 || order == 3) push_whole(0, whole.size());
+
+  std::vector<uint2> epi;
+  for (int b : tiled)
+    for (int l = 0; l < (grad ? h->d[b] : 1); ++l) epi.push_back(make_uint2((uint32_t)b, (uint32_t)l));
+
+  int rc;
+  if ((rc = grow(h->part, h->part_cap, part, "gradient partials"))) return rc;
+  if ((rc = grow(h->cpart, h->cpart_cap, cpart, "cost partials"))) return rc;
+  if ((rc = grow(h->task, h->task_cap, tasks.size(), "task queue"))) return rc;
+  if ((rc = grow(h->ctl, h->ctl_cap, 2 + (size_t)arrive, "counters"))) return rc;
+  if ((rc = grow(h->epi, h->epi_cap, epi.size(), "epilogue items"))) return rc;
+  hipStream_t st = iqsi::stream();
+  IQSI_TRY(hipMemcpyAsync(h->inst, I.data(), sizeof(HeaInst) * I.size(), hipMemcpyHostToDevice, st));
+  if (!tasks.empty())
+    IQSI_TRY(hipMemcpyAsync(h->task, tasks.data(), sizeof(HeaTask) * tasks.size(), hipMemcpyHostToDevice, st));
+  if (!epi.empty()) IQSI_TRY(hipMemcpyAsync(h->epi, epi.data(), sizeof(uint2) * epi.size(), hipMemcpyHostToDevice, st));
+  IQSI_TRY(hipStreamSynchronize(st));
+  h->ntask = (uint32_t)tasks.size();
+  h->nepi = (uint32_t)epi.size();
+  h->nctl = 2 + arrive;
+  h->built_mode = mode;
+  drop_graph(h);
+  return IQSH_OK;
+}
+
+template <typename T>
+int enqueue_t(iqsh_hea* h, int mode, hipStream_t st)
+{
+  HeaArgs<T> a;
+  a.inst = h->inst;
+  a.task = h->task;
+  a.ntask = h->ntask;
+  a.flags = (uint32_t)mode;
+  a.ctl = h->ctl;
+  a.arena = (V<T>*)h->arena;
+  a.mats = (const V<T>*)h->mats;
+  a.ang = h->ang;
+  a.grad = h->grad;
+  a.cost = h->cost;
+  a.part = h->part;
+  a.cpart = h->cpart;
+  a.spin_limit = kSpinLimit;
+  hipLaunchKernelGGL(k_hea_zero, dim3((h->nctl + 255) / 256), dim3(256), 0, st, h->ctl, h->nctl);
+  IQSI_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_hea_mats<T>, dim3((unsigned)((h->npairs + 255) / 256)), dim3(256), 0, st, (const double*)h->ang,
+                     (V<T>*)h->mats, h->npairs);
+  IQSI_TRY(hipGetLastError());
+  IQSI_TRY(launch_var<T>(h->tune.variant, a, h->tune.workers_per_cu, h->cus, st));
+  if (h->nepi) {
+    hipLaunchKernelGGL(k_hea_epi, dim3(h->nepi), dim3(64), 0, st, (const HeaInst*)h->inst, (const uint2*)h->epi,
+                       (const double*)h->ang, (const double*)h->part, (const double*)h->cpart, h->grad, h->cost,
+                       (int)(mode & kGrad));
+    IQSI_TRY(hipGetLastError());
+  }
+  return IQSH_OK;
+}
+
+int enqueue(iqsh_hea* h, int mode, hipStream_t st)
+{
+  return h->dtype == IQSH_C128 ? enqueue_t<double>(h, mode, st) : enqueue_t<float>(h, mode, st);
+}
+
+int check_handle(iqsh_hea* h) { return h ? IQSH_OK : fail(IQSH_ERR_ARG, "null batch handle"); }
+
+int check_tune(int dtype, const iqsh_hea_tune& t)
+{
+  if (t.variant < 0 || t.variant >= kNumVar) return fail(IQSH_ERR_ARG, "bad variant %d", t.variant);
+  if (t.workers_per_cu < 0 || t.workers_per_cu > 16) return fail(IQSH_ERR_ARG, "bad workers_per_cu %d", t.workers_per_cu);
+  if (t.order < 0 || t.order > 3) return fail(IQSH_ERR_ARG, "bad order %d", t.order);
+  (void)dtype;
+  return IQSH_OK;
+}
+
+}  // namespace
+
+int iqsh_hea_variants(int dtype, int* count)
+{
+  if (!count) return fail(IQSH_ERR_ARG, "null output");
+  if (dtype != IQSH_C64 && dtype != IQSH_C128) return fail(IQSH_ERR_ARG, "bad dtype %d", dtype);
+  *count = kNumVar;
+  return IQSH_OK;
+}
+
+int iqsh_hea_variant_info(int dtype, int v, int* tile_qubits, int* amps_per_thread, int* threads, int* for_tests)
+{
+  if (dtype != IQSH_C64 && dtype != IQSH_C128) return fail(IQSH_ERR_ARG, "bad dtype %d", dtype);
+  if (v < 0 || v >= kNumVar) return fail(IQSH_ERR_ARG, "bad variant %d", v);
+  const Variant& x = variant(dtype, v);
+  if (tile_qubits) *tile_qubits = x.M;
+  if (amps_per_thread) *amps_per_thread = x.R;
+  if (threads) *threads = (1 << x.M) / x.R;
+  if (for_tests) *for_tests = x.test;
+  return IQSH_OK;
+}
+
+int iqsh_hea_create(int dtype, int count, const int* nq, const int* depth, const iqsh_hea_tune* tune,
+                    iqsh_hea_handle* out)
+{
+  if (!out) return fail(IQSH_ERR_ARG, "null output handle");
+  *out = nullptr;
+  if (dtype != IQSH_C64 && dtype != IQSH_C128) return fail(IQSH_ERR_ARG, "bad dtype %d", dtype);
+  if (count < 1 || !nq || !depth) return fail(IQSH_ERR_ARG, "empty batch");
+  const iqsh_hea_tune t = tune ? *tune : iqsh_hea_tune{0, 0, 0};
+  if (int rc = check_tune(dtype, t)) return rc;
+  iqsh_hea* h = new iqsh_hea;
+  h->dtype = dtype;
+  h->count = count;
+  h->tune = t;
+  h->cus = std::max(1, iqsi::device_cus());
+  for (int b = 0; b < count; ++b) {
+    if (nq[b] < 1 || nq[b] > kMaxQubits || depth[b] < 1 || depth[b] > kMaxDepth) {
+      delete h;
+      return fail(IQSH_ERR_ARG, "instance %d: %d qubits, %d layers (need 1..%d qubits, 1..%d layers)", b, nq[b],
+                  depth[b], kMaxQubits, kMaxDepth);
+    }
+    h->n.push_back(nq[b]);
+    h->d.push_back(depth[b]);
+    h->par.push_back(h->npairs);
+    h->npairs += (uint64_t)nq[b] * depth[b];
+    h->soff.push_back(h->arena_amps);
+    h->arena_amps += uint64_t(2) << nq[b];  // psi and lam
+  }
+  auto bail = [&](int rc) {
+    iqsh_hea_destroy(h);
+    return rc;
+  };
+  auto alloc = [&](void** p, size_t bytes, const char* what) -> int {
+    hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 1));
+    return e == hipSuccess ? IQSH_OK
+                           : fail(IQSH_ERR_ALLOC, "hipMalloc(%zu bytes) for %s: %s", bytes, what, hipGetErrorString(e));
+  };
+  int rc;
+  if ((rc = alloc(&h->arena, h->arena_amps * h->esz(), "batch states"))) return bail(rc);
+  if ((rc = alloc(&h->mats, h->npairs * 4 * h->esz(), "gate matrices"))) return bail(rc);
+  if ((rc = alloc((void**)&h->ang, h->npairs * 2 * sizeof(double), "angles"))) return bail(rc);
+  if ((rc = alloc((void**)&h->grad, h->npairs * 2 * sizeof(double), "gradients"))) return bail(rc);
+  if ((rc = alloc((void**)&h->cost, (size_t)count * sizeof(double), "costs"))) return bail(rc);
+  if ((rc = alloc((void**)&h->inst, (size_t)count * sizeof(HeaInst), "instance table"))) return bail(rc);
+  if (hipMemset(h->ang, 0, h->npairs * 2 * sizeof(double)) != hipSuccess ||
+      hipMemset(h->grad, 0, h->npairs * 2 * sizeof(double)) != hipSuccess ||
+      hipMemset(h->cost, 0, (size_t)count * sizeof(double)) != hipSuccess)
+    return bail(fail(IQSH_ERR_HIP, "clearing batch buffers failed"));
+  if (hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(IQSH_ERR_HIP, "hipStreamCreateWithFlags failed"));
+  if ((rc = build(h, kGrad))) return bail(rc);
+  *out = h;
+  return IQSH_OK;
+}
+
+int iqsh_hea_destroy(iqsh_hea_handle h)
+{
+  if (!h) return IQSH_OK;
+  (void)hipStreamSynchronize(iqsi::stream());
+  drop_graph(h);
+  void* bufs[] = {h->arena, h->mats, h->ang, h->grad, h->cost, h->part, h->cpart, h->inst, h->task, h->ctl, h->epi};
+  for (void* p : bufs)
+    if (p) (void)hipFree(p);
+  if (h->cap) (void)hipStreamDestroy(h->cap);
+  delete h;
+  return IQSH_OK;
+}
+
+int iqsh_hea_num_params(iqsh_hea_handle h, uint64_t* np)
+{
+  if (int rc = check_handle(h)) return rc;
+  if (!np) return fail(IQSH_ERR_ARG, "null output");
+  *np = 2 * h->npairs;
+  return IQSH_OK;
+}
+
+int iqsh_hea_set_tune(iqsh_hea_handle h, const iqsh_hea_tune* t)
+{
+  if (int rc = check_handle(h)) return rc;
+  if (!t) return fail(IQSH_ERR_ARG, "null tune");
+  if (int rc = check_tune(h->dtype, *t)) return rc;
+  const bool same = t->variant == h->tune.variant && t->workers_per_cu == h->tune.workers_per_cu && t->order == h->tune.order;
+  if (same) return IQSH_OK;
+  const iqsh_hea_tune old = h->tune;
+  h->tune = *t;
+  const int rc = build(h, h->built_mode < 0 ? (int)kGrad : h->built_mode);
+  if (rc) {
+    h->tune = old;
+    build(h, h->built_mode < 0 ? (int)kGrad : h->built_mode);
+  }
+  return rc;
+}
+
+int iqsh_hea_get_tune(iqsh_hea_handle h, iqsh_hea_tune* t)
+{
+  if (int rc = check_handle(h)) return rc;
+  if (!t) return fail(IQSH_ERR_ARG, "null tune");
+  *t = h->tune;
+  return IQSH_OK;
+}
+
+int iqsh_hea_set_angles(iqsh_hea_handle h, const double* a)
+{
+  if (int rc = check_handle(h)) return rc;
+  if (!a) return fail(IQSH_ERR_ARG, "null angles");
+  IQSI_TRY(hipMemcpyAsync(h->ang, a, h->npairs * 2 * sizeof(double), hipMemcpyHostToDevice, iqsi::stream()));
+  IQSI_TRY(hipStreamSynchronize(iqsi::stream()));
+  return IQSH_OK;
+}
+
+int iqsh_hea_set_angles_device(iqsh_hea_handle h, const double* a)
+{
+  if (int rc = check_handle(h)) return rc;
+  if (!a) return fail(IQSH_ERR_ARG, "null angles");
+  IQSI_TRY(hipMemcpyAsync(h->ang, a, h->npairs * 2 * sizeof(double), hipMemcpyDeviceToDevice, iqsi::stream()));
+  return IQSH_OK;
+}
+
+int iqsh_hea_run(iqsh_hea_handle h, int flags)
+{
+  if (int rc = check_handle(h)) return rc;
+  if (flags & ~(IQSH_HEA_GRAD | IQSH_HEA_KEEP_STATE | IQSH_HEA_NO_GRAPH)) return fail(IQSH_ERR_ARG, "bad flags %d", flags);
+  if ((flags & kGrad) && (flags & kKeep)) return fail(IQSH_ERR_ARG, "IQSH_HEA_KEEP_STATE needs a forward-only run");
+  const int mode = flags & (kGrad | kKeep);
+  if (h->built_mode != mode)
+    if (int rc = build(h, mode)) return rc;
+  hipStream_t st = iqsi::stream();
+  if (flags & IQSH_HEA_NO_GRAPH) {
+    if (int rc = enqueue(h, mode, st)) return rc;
+  } else {
+    if (h->graph_mode != mode) {
+      drop_graph(h);
+      IQSI_TRY(hipStreamBeginCapture(h->cap, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue(h, mode, h->cap);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(h->cap, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(IQSH_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+      h->graph = g;
+      IQSI_TRY(hipGraphInstantiate(&h->gexec, h->graph, nullptr, nullptr, 0));
+      h->graph_mode = mode;
+    }
+    IQSI_TRY(hipGraphLaunch(h->gexec, st));
+  }
+  h->kept = (mode & kKeep) != 0;
+  return IQSH_OK;
+}
+
+int iqsh_hea_results(iqsh_hea_handle h, double* cost, double* grad)
+{
+  if (int rc = check_handle(h)) return rc;
+  hipStream_t st = iqsi::stream();
+  uint32_t ab = 0;
+  IQSI_TRY(hipMemcpyAsync(&ab, h->ctl + 1, sizeof ab, hipMemcpyDeviceToHost, st));
+  if (cost) IQSI_TRY(hipMemcpyAsync(cost, h->cost, (size_t)h->count * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (grad) IQSI_TRY(hipMemcpyAsync(grad, h->grad, h->npairs * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  IQSI_TRY(hipStreamSynchronize(st));
+  if (ab) return fail(IQSH_ERR_HIP, "batch run aborted: a tile dependency wait timed out");
+  return IQSH_OK;
+}
+
+int iqsh_hea_results_device(iqsh_hea_handle h, double* cost, double* grad)
+{
+  if (int rc = check_handle(h)) return rc;
+  hipStream_t st = iqsi::stream();
+  if (cost) IQSI_TRY(hipMemcpyAsync(cost, h->cost, (size_t)h->count * sizeof(double), hipMemcpyDeviceToDevice, st));
+  if (grad) IQSI_TRY(hipMemcpyAsync(grad, h->grad, h->npairs * 2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+  return IQSH_OK;
+}
+
+int iqsh_hea_state(iqsh_hea_handle h, int b, void* host)
+{
+  if (int rc = check_handle(h)) return rc;
+  if (b < 0 || b >= h->count || !host) return fail(IQSH_ERR_ARG, "bad instance %d or null buffer", b);
+  if (!h->kept) return fail(IQSH_ERR_ARG, "no state kept: run with IQSH_HEA_KEEP_STATE first");
+  hipStream_t st = iqsi::stream();
+  IQSI_TRY(hipMemcpyAsync(host, (const char*)h->arena + h->soff[b] * h->esz(), (size_t(1) << h->n[b]) * h->esz(),
+                          hipMemcpyDeviceToHost, st));
+  IQSI_TRY(hipStreamSynchronize(st));
+  return IQSH_OK;
+}
+
+int iqsh_hea_traffic(iqsh_hea_handle h, int flags, double* bytes)
+{
+  if (int rc = check_handle(h)) return rc;
+  if (!bytes) return fail(IQSH_ERR_ARG, "null output");
+  const Variant& v = variant(h->dtype, h->tune.variant);
+  const bool grad = flags & kGrad, keep = flags & kKeep;
+  double tot = 0;
+  for (int b = 0; b < h->count; ++b) {
+    const double S = (double)(uint64_t(1) << h->n[b]) * h->esz();
+    const int D = h->d[b];
+    if (h->n[b] <= v.M) {
+      if (keep) tot += S;
+      continue;
+    }
+    // Forward: step 0 writes psi, steps 1..D-1 read and write it.
+    tot += S + 2.0 * S * (D - 1);
+    if (grad) {
+      tot += 3.0 * S;                // turnaround: read psi, write psi and lam
+      tot += 4.0 * S * (D - 1);      // backward steps 1..D-1: read and write both
+      tot += 2.0 * S;                // last backward step: read both
+    } else {
+      tot += S + (keep ? S : 0.0);   // last forward step: read psi (and keep it)
+    }
+  }
+  *bytes = tot;
+  return IQSH_OK;
+}
+
+int iqsh_hea_autotune(iqsh_hea_handle h, int flags, int reps, iqsh_hea_trial* trials, int max_trials, int* ntrials,
+                      double* best_ms)
+{
+  if (int rc = check_handle(h)) return rc;
+  reps = std::max(1, reps);
+  const iqsh_hea_tune start = h->tune;
+  int nmax = 0;
+  for (int b = 0; b < h->count; ++b) nmax = std::max(nmax, h->n[b]);
+  hipEvent_t e0, e1;
+  IQSI_TRY(hipEventCreate(&e0));
+  IQSI_TRY(hipEventCreate(&e1));
+  hipStream_t st = iqsi::stream();
+  double best = 1e300;
+  iqsh_hea_tune bt = start;
+  int nt = 0, rc = IQSH_OK;
+  for (int v = 0; v < kNumVar && !rc; ++v) {
+    const Variant& x = variant(h->dtype, v);
+    if (x.test || nmax > 2 * x.M) continue;
+    for (int w = 1; w <= 2 && !rc; ++w) {
+      for (int o = 0; o < 4 && !rc; ++o) {
+        const iqsh_hea_tune t{v, w, o};
+        if ((rc = iqsh_hea_set_tune(h, &t))) break;
+        if ((rc = iqsh_hea_run(h, flags))) break;  // warm-up (and graph capture)
+        if (hipEventRecord(e0, st) != hipSuccess) { rc = fail(IQSH_ERR_HIP, "hipEventRecord failed"); break; }
+        for (int r = 0; r < reps && !rc; ++r) rc = iqsh_hea_run(h, flags);
+        if (rc) break;
+        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+          rc = fail(IQSH_ERR_HIP, "event timing failed");
+          break;
+        }
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if ((rc = iqsh_hea_results(h, nullptr, nullptr))) break;  // aborted runs are errors
+        const double per = ms / reps;
+        if (trials && nt < max_trials) trials[nt] = iqsh_hea_trial{t, per};
+        ++nt;
+        if (per < best) {
+          best = per;
+          bt = t;
+        }
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ntrials) *ntrials = nt;
+  if (rc) {
+    iqsh_hea_set_tune(h, &start);
+    return rc;
+  }
+  if (nt == 0) return fail(IQSH_ERR_NOTSUP, "no kernel variant fits instances of %d qubits", nmax);
+  if (best_ms) *best_ms = best;
+  return iqsh_hea_set_tune(h, &bt);
+}

@@ -13,6 +13,7 @@
 
 #include "../../include/iqs_hip.h"
 #include "iqs_kernels.hpp"
+#include "iqs_internal.hpp"
 
 using namespace iqsk;
 
@@ -738,6 +739,25 @@ int build_cut(iqsh_handle h, unsigned ne, const unsigned* u, const unsigned* v, 
 
 #define DISPATCH(h, CALL)                                   \
   ((h)->dtype == IQSH_C128 ? CALL<double> : CALL<float>)
+
+namespace iqsi {
+int fail(int code, const char* fmt, ...)
+{
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+hipStream_t stream() { return g_stream; }
+int device_cus()
+{
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  return cus;
+}
+}  // namespace iqsi
 
 // Definitions take C linkage from their declarations in iqs_hip.h.
 

@@ -12,6 +12,15 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libiqs_hip.so")
 
 C64, C128 = 0, 1
+HEA_GRAD, HEA_KEEP_STATE, HEA_NO_GRAPH = 1, 2, 4
+
+
+class HeaTune(C.Structure):
+    _fields_ = [("variant", C.c_int), ("workers_per_cu", C.c_int), ("order", C.c_int)]
+
+
+class HeaTrial(C.Structure):
+    _fields_ = [("tune", HeaTune), ("ms", C.c_double)]
 
 _lib = None
 
@@ -69,6 +78,21 @@ def _sig(L):
         "iqsh_qaoa_cost_apply": [P, P, U, P, P, P, DP],
         "iqsh_qaoa_phase_adjoint": [P, P, U, P, P, P, D, DP],
         "iqsh_adjoint_1q": [P, P, U, P, P, DP],
+        "iqsh_hea_variants": [I, C.POINTER(I)],
+        "iqsh_hea_variant_info": [I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(I)],
+        "iqsh_hea_create": [I, I, P, P, P, C.POINTER(P)],
+        "iqsh_hea_destroy": [P],
+        "iqsh_hea_num_params": [P, C.POINTER(U64)],
+        "iqsh_hea_set_tune": [P, P],
+        "iqsh_hea_get_tune": [P, P],
+        "iqsh_hea_set_angles": [P, P],
+        "iqsh_hea_set_angles_device": [P, P],
+        "iqsh_hea_run": [P, I],
+        "iqsh_hea_results": [P, P, P],
+        "iqsh_hea_results_device": [P, P, P],
+        "iqsh_hea_state": [P, I, P],
+        "iqsh_hea_traffic": [P, I, DP],
+        "iqsh_hea_autotune": [P, I, I, P, I, C.POINTER(I), DP],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

@@ -34,6 +34,14 @@ METRIC = "30q gate/s & %HBM peak; QAOA-28q grad s: adjoint vs param-shift @1e-6"
 HBM_PEAK_GBS = 8000.0
 
 
+_T0 = time.perf_counter()
+
+
+def log(msg: str) -> None:
+    """Progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"bench.py [{time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 def apply_op(sv, op, G):
     name = op[0]
     if name in G.NAMED_1Q:
@@ -238,10 +246,12 @@ def batch_measure(reps: int, torch):
            "params": int(2 * sum(n * d for n, d in zip(nq, dp)))}
     b64 = HeaBatch(nq, dp, "c64")
     b64.set_angles(ang)
+    log("HEA batch: autotuning complex64")
     t0 = time.perf_counter()
     tune, best_ms, trials = b64.autotune(grad=True, reps=2)
+    log(f"HEA batch: tuned {tune} at {best_ms:.2f} ms")
     out["autotune_s"] = time.perf_counter() - t0
-    out["tune"] = {"variant": tune[0], "workers_per_cu": tune[1], "order": tune[2],
+    out["tune"] = {"variant": tune[0], "workers_per_cu": tune[1], "order": tune[2], "group_mib": tune[3],
                    "tile": HeaBatch.variants("c64")[tune[0]]}
     out["trials_ms"] = [[list(t), round(ms, 3)] for t, ms in trials]
 
@@ -263,6 +273,7 @@ def batch_measure(reps: int, torch):
                   "instances_per_s": len(nq) / (ms_grad * 1e-3),
                   "algorithmic_bytes": traffic, "algorithmic_GBs": traffic / (ms_grad * 1e-3) / 1e9}
     b64.close()
+    log("HEA batch: complex128 reference run")
     b128 = HeaBatch(nq, dp, "c128")
     c128, g128 = b128.evaluate(ang)
     out["c128"] = {"ms_forward_and_gradient": timed(lambda: b128.run(grad=True), max(1, reps // 2))}
@@ -331,6 +342,7 @@ def main() -> int:
             apply_op(sv, op, G)
     sv.init_basis(0)
     torch.cuda.synchronize()
+    log(f"{n}-qubit register ready, {args.warmup} warm-up layers done")
 
     # ---- timed region: K layers, an event pair around every gate ----
     ev = []
@@ -354,6 +366,7 @@ def main() -> int:
     achieved = 2 * S / dense_avg / 1e9
 
     readout = [1.0 - 2.0 * sv.prob1(q) for q in range(n)]
+    log(f"timed brick circuit: {ngates} gates in {elapsed:.2f} s")
 
     # ---- per-kind sweep over every position ----
     sweep = {}
@@ -391,12 +404,18 @@ def main() -> int:
                 "max_GBs": nbytes / min(ts) / 1e9,
                 "per_target_GBs": [round(nbytes / t / 1e9, 1) for _, t in rows],
             }
+            log(f"sweep {kname}: {sweep[kname]['avg_GBs']:.0f} GB/s average")
 
     del sv
+    log("brick circuit and per-kind sweep done")
     qaoa = None if args.no_qaoa else qaoa_measure(args.qaoa_qubits, 8, args.qaoa_full, torch)
+    log("QAOA gradient done")
     fusion = None if args.no_fusion else fusion_measure(args.fusion_qubits, 200, torch)
+    log("fusion done")
     batch = None if args.no_batch else batch_measure(args.batch_reps, torch)
+    log("HEA batch done")
     cpu = None if args.no_cpu else cpu_baseline(n, layers[0])
+    log("CPU baseline done")
 
     out = {
         "metric": METRIC,

@@ -211,11 +211,15 @@ enum iqsh_hea_flags {
  * (0 = as many as fit); `order` the task order: 0 = tiled instances first, single-tile
  * instances last; 1 = single-tile first; 2 = single-tile instances interleaved among the
  * steps of the tiled ones; 3 = as 0 with every tiled instance's steps aligned to end on
- * the last round. */
+ * the last round.  `group_mib` > 0 processes the tiled instances in consecutive groups
+ * whose states (psi and lam) total at most that many MiB, so that a group's passes can be
+ * served from the last-level cache; single-tile instances are then spread over the groups
+ * and `order` is not used. */
 typedef struct {
   int variant;
   int workers_per_cu;
   int order;
+  int group_mib;
 } iqsh_hea_tune;
 
 typedef struct {
@@ -253,8 +257,9 @@ int iqsh_hea_state(iqsh_hea_handle h, int inst, void *host);
 /* Algorithmic HBM bytes of one run (amplitude reads and writes of the tiled instances;
  * single-tile instances stay on chip). */
 int iqsh_hea_traffic(iqsh_hea_handle h, int flags, double *bytes);
-/* Try every tuning (non-test variants that fit the batch x workers {1, 2} x orders
- * {0, 1, 2, 3}), `reps` timed runs each after one warm-up, keep the fastest.  Trials are
+/* Try every tuning (non-test variants that fit the batch x workers {1, 2} x (orders
+ * {0, 1, 2, 3} ungrouped, and groups of 64 and 160 MiB)), `reps` timed runs each after
+ * one warm-up, keep the fastest.  Trials are
  * written to `trials` (up to max_trials; *ntrials = how many were run). */
 int iqsh_hea_autotune(iqsh_hea_handle h, int flags, int reps, iqsh_hea_trial *trials,
                       int max_trials, int *ntrials, double *best_ms);

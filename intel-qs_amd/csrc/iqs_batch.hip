@@ -52,6 +52,8 @@ using iqsi::fail;
 namespace {
 
 template <typename T> using V = typename iqsk::Vec<T>::t;
+using iqsk::dcx;
+using iqsk::fcx;
 
 constexpr uint32_t kGrad = IQSH_HEA_GRAD, kKeep = IQSH_HEA_KEEP_STATE;
 
@@ -175,6 +177,34 @@ __device__ __forceinline__ void ud_apply(const C* u, C& a, C& b)
   r1.y = (u[1].x * a.y - u[1].y * a.x) + (u[3].x * b.y - u[3].y * b.x);
   a = r0;
   b = r1;
+}
+
+// Device-coherent amplitude accesses for tile passes.  Tiles of one step are produced and
+// consumed by workgroups on different XCDs, whose L2 caches are not coherent with each
+// other.  Agent-scope relaxed atomic loads and stores are coherent by themselves, so the
+// passes need no cache write-back or invalidation fences; ordering comes from waiting for
+// the stores to complete before the arrival counter is bumped.
+__device__ __forceinline__ fcx ld_coh(const fcx* p)
+{
+  const uint64_t v = __hip_atomic_load((uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __builtin_bit_cast(fcx, v);
+}
+__device__ __forceinline__ dcx ld_coh(const dcx* p)
+{
+  uint64_t* q = (uint64_t*)p;
+  const uint64_t a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint64_t b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return dcx{__builtin_bit_cast(double, a), __builtin_bit_cast(double, b)};
+}
+__device__ __forceinline__ void st_coh(fcx* p, const fcx& v)
+{
+  __hip_atomic_store((uint64_t*)p, __builtin_bit_cast(uint64_t, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coh(dcx* p, const dcx& v)
+{
+  uint64_t* q = (uint64_t*)p;
+  __hip_atomic_store(q, __builtin_bit_cast(uint64_t, v.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(q + 1, __builtin_bit_cast(uint64_t, v.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -413,11 +443,11 @@ struct Tile {
       }
     } else {
 #pragma unroll
-      for (int k = 0; k < R; ++k) x[k] = psi[g.gidx(lidx<LR>(tid, k, L))];
+      for (int k = 0; k < R; ++k) x[k] = ld_coh(psi + g.gidx(lidx<LR>(tid, k, L)));
     }
     if (s > D) {
 #pragma unroll
-      for (int k = 0; k < R; ++k) y[k] = lam[g.gidx(lidx<LR>(tid, k, L))];
+      for (int k = 0; k < R; ++k) y[k] = ld_coh(lam + g.gidx(lidx<LR>(tid, k, L)));
     }
     int cur = L;
     double* redw = red + (tid >> 6) * SLOTS;
@@ -436,11 +466,11 @@ struct Tile {
     }
     if (store_x) {
 #pragma unroll
-      for (int k = 0; k < R; ++k) psi[g.gidx(lidx<LR>(tid, k, L))] = x[k];
+      for (int k = 0; k < R; ++k) st_coh(psi + g.gidx(lidx<LR>(tid, k, L)), x[k]);
     }
     if (store_y) {
 #pragma unroll
-      for (int k = 0; k < R; ++k) lam[g.gidx(lidx<LR>(tid, k, L))] = y[k];
+      for (int k = 0; k < R; ++k) st_coh(lam + g.gidx(lidx<LR>(tid, k, L)), y[k]);
     }
     if (grad && s >= D) {
       __syncthreads();
@@ -460,7 +490,7 @@ struct Tile {
 };
 
 template <typename T, int M, int R>
-__global__ __launch_bounds__((Tile<T, M, R>::NT)) void k_hea(HeaArgs<T> A)
+__global__ __launch_bounds__((Tile<T, M, R>::NT), 2) void k_hea(HeaArgs<T> A)
 {
   using Tl = Tile<T, M, R>;
   __shared__ V<T> lds[1 << M];
@@ -468,17 +498,18 @@ __global__ __launch_bounds__((Tile<T, M, R>::NT)) void k_hea(HeaArgs<T> A)
   __shared__ double redc[Tl::NW];
   __shared__ uint32_t sh[2];
   const uint32_t tid = threadIdx.x;
+  // Thread 0 takes the next queue entry while the current task runs, so the atomic's round
+  // trip overlaps the work.
+  uint32_t next = 0;
+  if (tid == 0) next = atomicAdd(A.ctl, 1u);
   for (;;) {
-    if (tid == 0) {
-      uint32_t i = atomicAdd(A.ctl, 1u);
-      if (__hip_atomic_load(A.ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) i = A.ntask;
-      sh[0] = i;
-    }
+    if (tid == 0) sh[0] = next;
     __syncthreads();
     const uint32_t i = sh[0];
     if (i >= A.ntask) return;
     const HeaTask tk = A.task[i];
     const HeaInst I = A.inst[tk.inst];
+    if (tid == 0) next = atomicAdd(A.ctl, 1u);
     if (I.whole) {
       Tl::run_whole(A, I, tk.inst, lds, red, redc);
     } else {
@@ -488,7 +519,7 @@ __global__ __launch_bounds__((Tile<T, M, R>::NT)) void k_hea(HeaArgs<T> A)
           uint32_t* ctr = A.ctl + 2 + I.arrive + tk.step - 1;
           const uint32_t need = 1u << I.ntl;
           uint32_t spins = 0;
-          while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+          while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
             if (__hip_atomic_load(A.ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || ++spins > A.spin_limit) {
               __hip_atomic_store(A.ctl + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               ok = 0;
@@ -501,12 +532,11 @@ __global__ __launch_bounds__((Tile<T, M, R>::NT)) void k_hea(HeaArgs<T> A)
       }
       __syncthreads();
       if (sh[1]) {
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         Tl::run_pass(A, I, tk.step, tk.tile, lds, red, redc);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __builtin_amdgcn_s_waitcnt(0);  // this thread's state stores have completed
         __syncthreads();
         if (tid == 0)
-          __hip_atomic_fetch_add(A.ctl + 2 + I.arrive + tk.step, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(A.ctl + 2 + I.arrive + tk.step, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
     __syncthreads();
@@ -568,8 +598,10 @@ __global__ void k_hea_epi(const HeaInst* __restrict__ inst, const uint2* __restr
 struct Variant {
   int M, R, test;
 };
-constexpr Variant kVar64[] = {{13, 32, 0}, {13, 16, 0}, {12, 16, 0}, {8, 4, 1}};
-constexpr Variant kVar128[] = {{12, 16, 0}, {12, 8, 0}, {11, 8, 0}, {8, 4, 1}};
+// Every variant keeps its registers within 256 VGPRs (two waves per SIMD); tiles that
+// needed the accumulation registers as spill space ran at one wave per SIMD and are left out.
+constexpr Variant kVar64[] = {{13, 16, 0}, {12, 16, 0}, {12, 8, 0}, {8, 4, 1}};
+constexpr Variant kVar128[] = {{12, 8, 0}, {11, 8, 0}, {10, 8, 0}, {8, 4, 1}};
 constexpr int kNumVar = 4;
 
 const Variant& variant(int dtype, int v) { return dtype == IQSH_C128 ? kVar128[v] : kVar64[v]; }
@@ -593,9 +625,9 @@ template <>
 hipError_t launch_var<float>(int v, const HeaArgs<float>& a, int wpc, int cus, hipStream_t st)
 {
   switch (v) {
-    case 0: return launch_t<float, 13, 32>(a, wpc, cus, st);
-    case 1: return launch_t<float, 13, 16>(a, wpc, cus, st);
-    case 2: return launch_t<float, 12, 16>(a, wpc, cus, st);
+    case 0: return launch_t<float, 13, 16>(a, wpc, cus, st);
+    case 1: return launch_t<float, 12, 16>(a, wpc, cus, st);
+    case 2: return launch_t<float, 12, 8>(a, wpc, cus, st);
     default: return launch_t<float, 8, 4>(a, wpc, cus, st);
   }
 }
@@ -603,9 +635,9 @@ template <>
 hipError_t launch_var<double>(int v, const HeaArgs<double>& a, int wpc, int cus, hipStream_t st)
 {
   switch (v) {
-    case 0: return launch_t<double, 12, 16>(a, wpc, cus, st);
-    case 1: return launch_t<double, 12, 8>(a, wpc, cus, st);
-    case 2: return launch_t<double, 11, 8>(a, wpc, cus, st);
+    case 0: return launch_t<double, 12, 8>(a, wpc, cus, st);
+    case 1: return launch_t<double, 11, 8>(a, wpc, cus, st);
+    case 2: return launch_t<double, 10, 8>(a, wpc, cus, st);
     default: return launch_t<double, 8, 4>(a, wpc, cus, st);
   }
 }
@@ -622,7 +654,7 @@ struct iqsh_hea {
   std::vector<int> n, d;
   std::vector<uint64_t> par, soff;
   uint64_t npairs = 0, arena_amps = 0;
-  iqsh_hea_tune tune{0, 0, 0};
+  iqsh_hea_tune tune{0, 0, 0, 0};
   int cus = 0;
   hipStream_t cap = nullptr;  // private stream for graph capture
 
@@ -730,24 +762,53 @@ int build(iqsh_hea* h, int mode)
   auto push_whole = [&](size_t lo, size_t hi) {
     for (size_t i = lo; i < hi; ++i) tasks.push_back(HeaTask{(uint32_t)whole[i], 0, 0});
   };
-  if (order == 1) push_whole(0, whole.size());
-  size_t wnext = 0;
-  for (int r = 0; r < smax; ++r) {
+  // Rounds of a set of tiled instances: round r holds step r - offset of each instance.
+  auto push_rounds = [&](const std::vector<int>& set, bool align_end, size_t w_lo, size_t w_hi) {
+    int sm = 0;
+    for (int b : set) sm = std::max(sm, steps_of(h->d[b], grad));
+    size_t wn = w_lo;
+    for (int r = 0; r < sm; ++r) {
+      for (int b : set) {
+        const int S = steps_of(h->d[b], grad);
+        const int s = r - (align_end ? sm - S : 0);
+        if (s < 0 || s >= S) continue;
+        const uint32_t T = 1u << I[b].ntl;
+        for (uint32_t t = 0; t < T; ++t) tasks.push_back(HeaTask{(uint32_t)b, (uint16_t)s, (uint16_t)t});
+      }
+      const size_t upto = w_lo + (w_hi - w_lo) * (size_t)(r + 1) / (size_t)std::max(sm, 1);
+      push_whole(wn, upto);
+      wn = upto;
+    }
+    push_whole(wn, w_hi);
+  };
+  if (h->tune.group_mib > 0) {
+    // Groups of tiled instances (largest first) within the byte budget; single-tile
+    // instances spread evenly over the groups' rounds.
+    const double budget = (double)h->tune.group_mib * 1048576.0;
+    std::vector<std::vector<int>> groups;
+    double bytes = 0;
     for (int b : tiled) {
-      const int S = steps_of(h->d[b], grad);
-      const int s = r - (order == 3 ? smax - S : 0);
-      if (s < 0 || s >= S) continue;
-      const uint32_t T = 1u << I[b].ntl;
-      for (uint32_t t = 0; t < T; ++t) tasks.push_back(HeaTask{(uint32_t)b, (uint16_t)s, (uint16_t)t});
+      const double sb = 2.0 * (double)(uint64_t(1) << h->n[b]) * h->esz();
+      if (groups.empty() || (bytes + sb > budget && !groups.back().empty())) {
+        groups.emplace_back();
+        bytes = 0;
+      }
+      groups.back().push_back(b);
+      bytes += sb;
     }
-    if (order == 2) {
-      const size_t upto = whole.size() * (size_t)(r + 1) / (size_t)smax;
-      push_whole(wnext, upto);
-      wnext = upto;
-    }
+    const size_t G = std::max<size_t>(groups.size(), 1);
+    for (size_t gi = 0; gi < groups.size(); ++gi)
+      push_rounds(groups[gi], false, whole.size() * gi / G, whole.size() * (gi + 1) / G);
+    if (groups.empty()) push_whole(0, whole.size());
+  } else {
+    if (order == 1) push_whole(0, whole.size());
+    if (order == 2)
+      push_rounds(tiled, false, 0, whole.size());
+    else
+      push_rounds(tiled, order == 3, 0, 0);
+    if (order == 0 || order == 3) push_whole(0, whole.size());
   }
-  if (order == 2) push_whole(wnext, whole.size());
-  if (order == 0 || order == 3) push_whole(0, whole.size());
+  (void)smax;
 
   std::vector<uint2> epi;
   for (int b : tiled)
@@ -817,6 +878,7 @@ int check_tune(int dtype, const iqsh_hea_tune& t)
   if (t.variant < 0 || t.variant >= kNumVar) return fail(IQSH_ERR_ARG, "bad variant %d", t.variant);
   if (t.workers_per_cu < 0 || t.workers_per_cu > 16) return fail(IQSH_ERR_ARG, "bad workers_per_cu %d", t.workers_per_cu);
   if (t.order < 0 || t.order > 3) return fail(IQSH_ERR_ARG, "bad order %d", t.order);
+  if (t.group_mib < 0 || t.group_mib > (1 << 20)) return fail(IQSH_ERR_ARG, "bad group_mib %d", t.group_mib);
   (void)dtype;
   return IQSH_OK;
 }
@@ -850,7 +912,7 @@ int iqsh_hea_create(int dtype, int count, const int* nq, const int* depth, const
   *out = nullptr;
   if (dtype != IQSH_C64 && dtype != IQSH_C128) return fail(IQSH_ERR_ARG, "bad dtype %d", dtype);
   if (count < 1 || !nq || !depth) return fail(IQSH_ERR_ARG, "empty batch");
-  const iqsh_hea_tune t = tune ? *tune : iqsh_hea_tune{0, 0, 0};
+  const iqsh_hea_tune t = tune ? *tune : iqsh_hea_tune{0, 0, 0, 0};
   if (int rc = check_tune(dtype, t)) return rc;
   iqsh_hea* h = new iqsh_hea;
   h->dtype = dtype;
@@ -923,7 +985,8 @@ int iqsh_hea_set_tune(iqsh_hea_handle h, const iqsh_hea_tune* t)
   if (int rc = check_handle(h)) return rc;
   if (!t) return fail(IQSH_ERR_ARG, "null tune");
   if (int rc = check_tune(h->dtype, *t)) return rc;
-  const bool same = t->variant == h->tune.variant && t->workers_per_cu == h->tune.workers_per_cu && t->order == h->tune.order;
+  const bool same = t->variant == h->tune.variant && t->workers_per_cu == h->tune.workers_per_cu &&
+                    t->order == h->tune.order && t->group_mib == h->tune.group_mib;
   if (same) return IQSH_OK;
   const iqsh_hea_tune old = h->tune;
   h->tune = *t;
@@ -1074,8 +1137,8 @@ int iqsh_hea_autotune(iqsh_hea_handle h, int flags, int reps, iqsh_hea_trial* tr
     const Variant& x = variant(h->dtype, v);
     if (x.test || nmax > 2 * x.M) continue;
     for (int w = 1; w <= 2 && !rc; ++w) {
-      for (int o = 0; o < 4 && !rc; ++o) {
-        const iqsh_hea_tune t{v, w, o};
+      for (int o = 0; o < 6 && !rc; ++o) {
+        const iqsh_hea_tune t = o < 4 ? iqsh_hea_tune{v, w, o, 0} : iqsh_hea_tune{v, w, 0, o == 4 ? 64 : 160};
         if ((rc = iqsh_hea_set_tune(h, &t))) break;
         if ((rc = iqsh_hea_run(h, flags))) break;  // warm-up (and graph capture)
         if (hipEventRecord(e0, st) != hipSuccess) { rc = fail(IQSH_ERR_HIP, "hipEventRecord failed"); break; }

@@ -16,7 +16,7 @@ HEA_GRAD, HEA_KEEP_STATE, HEA_NO_GRAPH = 1, 2, 4
 
 
 class HeaTune(C.Structure):
-    _fields_ = [("variant", C.c_int), ("workers_per_cu", C.c_int), ("order", C.c_int)]
+    _fields_ = [("variant", C.c_int), ("workers_per_cu", C.c_int), ("order", C.c_int), ("group_mib", C.c_int)]
 
 
 class HeaTrial(C.Structure):
@@ -105,6 +105,17 @@ def _sig(L):
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
+        # One HIP runtime per process: PyTorch ships its own libamdhip64.so.7 under the same
+        # soname as the system runtime this library links, and the first one loaded serves
+        # every later user.  PyTorch's GPU set-up fails ("No HIP GPUs are available") once
+        # this library has used the runtime first, so when PyTorch is installed and sees a
+        # GPU, its set-up runs here, before the first call into this library.
+        try:
+            import torch
+        except ImportError:
+            torch = None
+        if torch is not None and torch.cuda.is_available():
+            torch.cuda.init()
         if not os.path.exists(LIB_PATH):
             raise IqsError(f"HIP engine library missing: {LIB_PATH} (build with `make -C intel-qs_amd`)")
         L = C.CDLL(LIB_PATH)

@@ -66,7 +66,7 @@ class HeaBatch:
     """A batch of HEA instances resident on the GPU (states, matrices, work buffers)."""
 
     def __init__(self, nqubits: Sequence[int], depths: Sequence[int], dtype: str = "c64",
-                 tune: Optional[Tuple[int, int, int]] = None):
+                 tune: Optional[Tuple[int, ...]] = None):
         self.nqubits = [int(x) for x in nqubits]
         self.depths = [int(x) for x in depths]
         self.count = len(self.nqubits)
@@ -108,13 +108,14 @@ class HeaBatch:
         return out
 
     @property
-    def tune(self) -> Tuple[int, int, int]:
+    def tune(self) -> Tuple[int, int, int, int]:
+        """(variant, workers_per_cu, order, group_mib), see iqsh_hea_tune."""
         t = HeaTune()
         call("iqsh_hea_get_tune", self.h, C.byref(t))
-        return (t.variant, t.workers_per_cu, t.order)
+        return (t.variant, t.workers_per_cu, t.order, t.group_mib)
 
     @tune.setter
-    def tune(self, value: Tuple[int, int, int]) -> None:
+    def tune(self, value: Tuple[int, ...]) -> None:
         t = HeaTune(*value)
         call("iqsh_hea_set_tune", self.h, C.byref(t))
 
@@ -123,7 +124,8 @@ class HeaBatch:
         trials = (HeaTrial * 64)()
         nt, best = C.c_int(), C.c_double()
         call("iqsh_hea_autotune", self.h, HEA_GRAD if grad else 0, reps, trials, 64, C.byref(nt), C.byref(best))
-        tr = [((x.tune.variant, x.tune.workers_per_cu, x.tune.order), x.ms) for x in trials[:min(nt.value, 64)]]
+        tr = [((x.tune.variant, x.tune.workers_per_cu, x.tune.order, x.tune.group_mib), x.ms)
+              for x in trials[:min(nt.value, 64)]]
         return self.tune, best.value, tr
 
     # ---- data ----

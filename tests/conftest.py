@@ -9,3 +9,12 @@ for p in (ROOT, os.path.join(ROOT, "intel-qs_amd"), os.path.join(ROOT, "oracle")
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP path)")
+    # One HIP runtime per process (PyTorch bundles its own under the system runtime's
+    # soname): PyTorch's GPU set-up runs before any test module loads an engine library,
+    # the order bench.py uses.
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.cuda.init()

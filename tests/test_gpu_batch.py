@@ -127,6 +127,10 @@ def test_tunings_agree():
             b.tune = (TEST_VARIANT, w, o)
             c, g = b.evaluate(ang)
             assert np.array_equal(c, c0) and np.array_equal(g, g0), (w, o)
+        for gm in (1, 4):  # groups of a few instances (states of 2^16 x 16 B = 1 MiB each)
+            b.tune = (TEST_VARIANT, w, 0, gm)
+            c, g = b.evaluate(ang)
+            assert np.array_equal(c, c0) and np.array_equal(g, g0), (w, gm)
     for v in range(3):
         b.tune = (v, 0, 0)
         c, g = b.evaluate(ang)
@@ -140,7 +144,7 @@ def test_autotune_small_batch():
     b.set_angles(ang)
     c0, g0 = b.evaluate(ang)
     tune, ms, trials = b.autotune(reps=1)
-    assert len(trials) == 3 * 2 * 4 and ms > 0 and tune in [t for t, _ in trials]
+    assert len(trials) == 3 * 2 * 6 and ms > 0 and tune in [t for t, _ in trials]
     c, g = b.evaluate(ang)
     assert np.max(np.abs(c - c0)) < 1e-5 and np.max(np.abs(g - g0)) < 1e-5
     b.close()

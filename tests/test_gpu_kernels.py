@@ -424,3 +424,33 @@ def test_block5_mfma_vs_valu_and_gather(dt, positions):
     t = 1e-12 if dt == "c128" else 3e-5
     assert np.abs(got - want).max() < t
     assert np.abs(valu - want).max() < t
+
+
+@pytest.mark.parametrize("positions", [[0, 1, 2], [13, 2, 7], [12, 13, 11], [1, 3, 5, 7], [12, 13, 0, 6], [3, 2, 1, 0]])
+def test_block34_padded_to_mfma(dt, positions):
+    """3- and 4-qubit blocks run as U (x) I on the matrix cores (padded with the highest
+    free positions): compare with the vector-ALU kernel and an explicit gather."""
+    from iqs_amd._lib import call
+    n, k = 14, len(positions)
+    U = haar_unitary(1 << k, SplitMix64(7 * sum(positions) + k))
+    h = rand_state(n, dt, 43)
+    sv = on_gpu(h, dt)
+    sv.apply_kq(positions, U)
+    got = sv.download()
+    sv.upload(h)
+    call("iqsh_set_block_mfma", 0)
+    try:
+        sv.apply_kq(positions, U)
+    finally:
+        call("iqsh_set_block_mfma", 1)
+    valu = sv.download()
+    a = h.astype(np.complex128)
+    want = a.copy()
+    j = np.arange(1 << n)
+    mask = sum(1 << p for p in positions)
+    bases = j[(j & mask) == 0]
+    idx = np.stack([bases | sum(((r >> b) & 1) << positions[b] for b in range(k)) for r in range(1 << k)])
+    want[idx] = U @ a[idx]
+    t = 1e-12 if dt == "c128" else 3e-5
+    assert np.abs(got - want).max() < t
+    assert np.abs(valu - want).max() < t

@@ -52,8 +52,6 @@ using iqsi::fail;
 namespace {
 
 template <typename T> using V = typename iqsk::Vec<T>::t;
-using iqsk::dcx;
-using iqsk::fcx;
 
 constexpr uint32_t kGrad = IQSH_HEA_GRAD, kKeep = IQSH_HEA_KEEP_STATE;
 
@@ -75,12 +73,17 @@ struct HeaTask {
   uint16_t tile;
 };
 
+// One task queue per XCD (see k_hea).
+constexpr int kQueues = 8;
+constexpr int kCtlAbort = kQueues;       // ctl[0..7]: queue heads, ctl[8]: abort flag
+constexpr int kCtlArrive = kQueues + 1;  // ctl[9 + i]: arrival counter i
+
 template <typename T> struct HeaArgs {
   const HeaInst* inst;
-  const HeaTask* task;
-  uint32_t ntask;
+  const HeaTask* task;  // the queues, back to back
+  const uint32_t* qoff;  // queue q = task[qoff[q] .. qoff[q + 1])
   uint32_t flags;
-  uint32_t* ctl;  // [0] queue head, [1] abort flag, [2 + i] arrival counter i
+  uint32_t* ctl;
   V<T>* arena;
   const V<T>* mats;  // 4 entries per pair: U = RZ(phi) RY(theta), row-major
   const double* ang;
@@ -179,33 +182,16 @@ __device__ __forceinline__ void ud_apply(const C* u, C& a, C& b)
   b = r1;
 }
 
-// Device-coherent amplitude accesses for tile passes.  Tiles of one step are produced and
-// consumed by workgroups on different XCDs, whose L2 caches are not coherent with each
-// other.  Agent-scope relaxed atomic loads and stores are coherent by themselves, so the
-// passes need no cache write-back or invalidation fences; ordering comes from waiting for
-// the stores to complete before the arrival counter is bumped.
-__device__ __forceinline__ fcx ld_coh(const fcx* p)
-{
-  const uint64_t v = __hip_atomic_load((uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return __builtin_bit_cast(fcx, v);
-}
-__device__ __forceinline__ dcx ld_coh(const dcx* p)
-{
-  uint64_t* q = (uint64_t*)p;
-  const uint64_t a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const uint64_t b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return dcx{__builtin_bit_cast(double, a), __builtin_bit_cast(double, b)};
-}
-__device__ __forceinline__ void st_coh(fcx* p, const fcx& v)
-{
-  __hip_atomic_store((uint64_t*)p, __builtin_bit_cast(uint64_t, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_coh(dcx* p, const dcx& v)
-{
-  uint64_t* q = (uint64_t*)p;
-  __hip_atomic_store(q, __builtin_bit_cast(uint64_t, v.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(q + 1, __builtin_bit_cast(uint64_t, v.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// Coherence of tile passes.  All tiles of one register are produced and consumed by
+// workgroups of the same XCD (one queue per XCD, chosen by the XCC_ID hardware register):
+// a producer waits for its stores to complete before bumping the arrival counter, and a
+// consumer performs an agent-scope acquire (buffer_inv sc1) before loading.  Measured with
+// a message-passing test on MI355X (1024 workgroups, 20 rounds per case): same XCD with the
+// acquire, 0 stale reads; same XCD with only an L1 invalidate (buffer_inv sc0), every read
+// stale; different XCDs without an L2 write-back, every read stale.  Sharing tiles across
+// XCDs would need the release's L2 write-back (buffer_wbl2) around every task, which made
+// the passes 4x slower.
+__device__ __forceinline__ void acquire_tiles() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -443,11 +429,11 @@ struct Tile {
       }
     } else {
 #pragma unroll
-      for (int k = 0; k < R; ++k) x[k] = ld_coh(psi + g.gidx(lidx<LR>(tid, k, L)));
+      for (int k = 0; k < R; ++k) x[k] = psi[g.gidx(lidx<LR>(tid, k, L))];
     }
     if (s > D) {
 #pragma unroll
-      for (int k = 0; k < R; ++k) y[k] = ld_coh(lam + g.gidx(lidx<LR>(tid, k, L)));
+      for (int k = 0; k < R; ++k) y[k] = lam[g.gidx(lidx<LR>(tid, k, L))];
     }
     int cur = L;
     double* redw = red + (tid >> 6) * SLOTS;
@@ -466,11 +452,11 @@ struct Tile {
     }
     if (store_x) {
 #pragma unroll
-      for (int k = 0; k < R; ++k) st_coh(psi + g.gidx(lidx<LR>(tid, k, L)), x[k]);
+      for (int k = 0; k < R; ++k) psi[g.gidx(lidx<LR>(tid, k, L))] = x[k];
     }
     if (store_y) {
 #pragma unroll
-      for (int k = 0; k < R; ++k) st_coh(lam + g.gidx(lidx<LR>(tid, k, L)), y[k]);
+      for (int k = 0; k < R; ++k) lam[g.gidx(lidx<LR>(tid, k, L))] = y[k];
     }
     if (grad && s >= D) {
       __syncthreads();
@@ -498,30 +484,36 @@ __global__ __launch_bounds__((Tile<T, M, R>::NT), 2) void k_hea(HeaArgs<T> A)
   __shared__ double redc[Tl::NW];
   __shared__ uint32_t sh[2];
   const uint32_t tid = threadIdx.x;
+  // The queue of this workgroup's XCD (hardware register XCC_ID, bits 3:0).
+  const uint32_t q = __builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)) % kQueues;
+  const HeaTask* task = A.task + A.qoff[q];
+  const uint32_t ntask = A.qoff[q + 1] - A.qoff[q];
+  uint32_t* head = A.ctl + q;
   // Thread 0 takes the next queue entry while the current task runs, so the atomic's round
   // trip overlaps the work.
   uint32_t next = 0;
-  if (tid == 0) next = atomicAdd(A.ctl, 1u);
+  if (tid == 0) next = atomicAdd(head, 1u);
   for (;;) {
     if (tid == 0) sh[0] = next;
     __syncthreads();
     const uint32_t i = sh[0];
-    if (i >= A.ntask) return;
-    const HeaTask tk = A.task[i];
+    if (i >= ntask) return;
+    const HeaTask tk = task[i];
     const HeaInst I = A.inst[tk.inst];
-    if (tid == 0) next = atomicAdd(A.ctl, 1u);
+    if (tid == 0) next = atomicAdd(head, 1u);
     if (I.whole) {
       Tl::run_whole(A, I, tk.inst, lds, red, redc);
     } else {
       if (tid == 0) {
         uint32_t ok = 1;
         if (tk.step > 0) {
-          uint32_t* ctr = A.ctl + 2 + I.arrive + tk.step - 1;
+          uint32_t* ctr = A.ctl + kCtlArrive + I.arrive + tk.step - 1;
           const uint32_t need = 1u << I.ntl;
           uint32_t spins = 0;
           while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
-            if (__hip_atomic_load(A.ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || ++spins > A.spin_limit) {
-              __hip_atomic_store(A.ctl + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (__hip_atomic_load(A.ctl + kCtlAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                ++spins > A.spin_limit) {
+              __hip_atomic_store(A.ctl + kCtlAbort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               ok = 0;
               break;
             }
@@ -532,11 +524,13 @@ __global__ __launch_bounds__((Tile<T, M, R>::NT), 2) void k_hea(HeaArgs<T> A)
       }
       __syncthreads();
       if (sh[1]) {
+        if (tk.step > 0) acquire_tiles();
         Tl::run_pass(A, I, tk.step, tk.tile, lds, red, redc);
         __builtin_amdgcn_s_waitcnt(0);  // this thread's state stores have completed
         __syncthreads();
         if (tid == 0)
-          __hip_atomic_fetch_add(A.ctl + 2 + I.arrive + tk.step, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_fetch_add(A.ctl + kCtlArrive + I.arrive + tk.step, 1u, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
       }
     }
     __syncthreads();
@@ -672,6 +666,9 @@ struct iqsh_hea {
   size_t task_cap = 0;
   uint32_t* ctl = nullptr;
   size_t ctl_cap = 0;
+  uint32_t* qoff = nullptr;
+  size_t qoff_cap = 0;
+  std::vector<uint32_t> qlen;  // tasks per queue
   uint2* epi = nullptr;
   size_t epi_cap = 0;
 
@@ -757,57 +754,86 @@ int build(iqsh_hea* h, int mode)
   std::stable_sort(tiled.begin(), tiled.end(), by_size);
   std::stable_sort(whole.begin(), whole.end(), by_size);
 
-  std::vector<HeaTask> tasks;
-  const int order = h->tune.order;
-  auto push_whole = [&](size_t lo, size_t hi) {
-    for (size_t i = lo; i < hi; ++i) tasks.push_back(HeaTask{(uint32_t)whole[i], 0, 0});
+  // Every register is served by one XCD's queue (its tiles share that XCD's L2).  Registers
+  // are dealt longest first to the least loaded queue; the cost is the bytes a register's
+  // passes move (single-tile registers stay on chip: weighted by half their padded size).
+  auto cost = [&](int b) {
+    const double esz = (double)h->esz();
+    if (I[b].whole) return 0.5 * h->d[b] * (double)(1u << v.M) * esz * (grad ? 3.0 : 1.0);
+    return (double)steps_of(h->d[b], grad) * (double)(uint64_t(1) << h->n[b]) * esz * 3.0;
   };
-  // Rounds of a set of tiled instances: round r holds step r - offset of each instance.
-  auto push_rounds = [&](const std::vector<int>& set, bool align_end, size_t w_lo, size_t w_hi) {
-    int sm = 0;
-    for (int b : set) sm = std::max(sm, steps_of(h->d[b], grad));
-    size_t wn = w_lo;
-    for (int r = 0; r < sm; ++r) {
-      for (int b : set) {
-        const int S = steps_of(h->d[b], grad);
-        const int s = r - (align_end ? sm - S : 0);
-        if (s < 0 || s >= S) continue;
-        const uint32_t T = 1u << I[b].ntl;
-        for (uint32_t t = 0; t < T; ++t) tasks.push_back(HeaTask{(uint32_t)b, (uint16_t)s, (uint16_t)t});
-      }
-      const size_t upto = w_lo + (w_hi - w_lo) * (size_t)(r + 1) / (size_t)std::max(sm, 1);
-      push_whole(wn, upto);
-      wn = upto;
-    }
-    push_whole(wn, w_hi);
-  };
-  if (h->tune.group_mib > 0) {
-    // Groups of tiled instances (largest first) within the byte budget; single-tile
-    // instances spread evenly over the groups' rounds.
-    const double budget = (double)h->tune.group_mib * 1048576.0;
-    std::vector<std::vector<int>> groups;
-    double bytes = 0;
-    for (int b : tiled) {
-      const double sb = 2.0 * (double)(uint64_t(1) << h->n[b]) * h->esz();
-      if (groups.empty() || (bytes + sb > budget && !groups.back().empty())) {
-        groups.emplace_back();
-        bytes = 0;
-      }
-      groups.back().push_back(b);
-      bytes += sb;
-    }
-    const size_t G = std::max<size_t>(groups.size(), 1);
-    for (size_t gi = 0; gi < groups.size(); ++gi)
-      push_rounds(groups[gi], false, whole.size() * gi / G, whole.size() * (gi + 1) / G);
-    if (groups.empty()) push_whole(0, whole.size());
-  } else {
-    if (order == 1) push_whole(0, whole.size());
-    if (order == 2)
-      push_rounds(tiled, false, 0, whole.size());
-    else
-      push_rounds(tiled, order == 3, 0, 0);
-    if (order == 0 || order == 3) push_whole(0, whole.size());
+  std::vector<int> all(h->count);
+  for (int b = 0; b < h->count; ++b) all[b] = b;
+  std::stable_sort(all.begin(), all.end(), [&](int a, int b) { return cost(a) > cost(b); });
+  std::vector<int> qof(h->count);
+  double load[kQueues] = {0};
+  for (int b : all) {
+    const int qi = (int)(std::min_element(load, load + kQueues) - load);
+    qof[b] = qi;
+    load[qi] += cost(b);
   }
+
+  std::vector<HeaTask> tasks;
+  std::vector<uint32_t> qoff(kQueues + 1, 0);
+  const int order = h->tune.order;
+  for (int qi = 0; qi < kQueues; ++qi) {
+    qoff[qi] = (uint32_t)tasks.size();
+    std::vector<int> tq, wq;
+    for (int b : tiled)
+      if (qof[b] == qi) tq.push_back(b);
+    for (int b : whole)
+      if (qof[b] == qi) wq.push_back(b);
+    auto push_whole = [&](size_t lo, size_t hi) {
+      for (size_t i = lo; i < hi; ++i) tasks.push_back(HeaTask{(uint32_t)wq[i], 0, 0});
+    };
+    // Rounds of a set of tiled registers: round r holds step r - offset of each register.
+    auto push_rounds = [&](const std::vector<int>& set, bool align_end, size_t w_lo, size_t w_hi) {
+      int sm = 0;
+      for (int b : set) sm = std::max(sm, steps_of(h->d[b], grad));
+      size_t wn = w_lo;
+      for (int r = 0; r < sm; ++r) {
+        for (int b : set) {
+          const int S = steps_of(h->d[b], grad);
+          const int st = r - (align_end ? sm - S : 0);
+          if (st < 0 || st >= S) continue;
+          const uint32_t T = 1u << I[b].ntl;
+          for (uint32_t t = 0; t < T; ++t) tasks.push_back(HeaTask{(uint32_t)b, (uint16_t)st, (uint16_t)t});
+        }
+        const size_t upto = w_lo + (w_hi - w_lo) * (size_t)(r + 1) / (size_t)std::max(sm, 1);
+        push_whole(wn, upto);
+        wn = upto;
+      }
+      push_whole(wn, w_hi);
+    };
+    if (h->tune.group_mib > 0) {
+      // Groups of tiled registers (largest first) within this queue's share of the byte
+      // budget; single-tile registers spread evenly over the groups' rounds.
+      const double budget = (double)h->tune.group_mib * 1048576.0 / kQueues;
+      std::vector<std::vector<int>> groups;
+      double bytes = 0;
+      for (int b : tq) {
+        const double sb = 2.0 * (double)(uint64_t(1) << h->n[b]) * h->esz();
+        if (groups.empty() || (bytes + sb > budget && !groups.back().empty())) {
+          groups.emplace_back();
+          bytes = 0;
+        }
+        groups.back().push_back(b);
+        bytes += sb;
+      }
+      const size_t G = std::max<size_t>(groups.size(), 1);
+      for (size_t gi = 0; gi < groups.size(); ++gi)
+        push_rounds(groups[gi], false, wq.size() * gi / G, wq.size() * (gi + 1) / G);
+      if (groups.empty()) push_whole(0, wq.size());
+    } else {
+      if (order == 1) push_whole(0, wq.size());
+      if (order == 2)
+        push_rounds(tq, false, 0, wq.size());
+      else
+        push_rounds(tq, order == 3, 0, 0);
+      if (order == 0 || order == 3) push_whole(0, wq.size());
+    }
+  }
+  qoff[kQueues] = (uint32_t)tasks.size();
   (void)smax;
 
   std::vector<uint2> epi;
@@ -818,17 +844,21 @@ int build(iqsh_hea* h, int mode)
   if ((rc = grow(h->part, h->part_cap, part, "gradient partials"))) return rc;
   if ((rc = grow(h->cpart, h->cpart_cap, cpart, "cost partials"))) return rc;
   if ((rc = grow(h->task, h->task_cap, tasks.size(), "task queue"))) return rc;
-  if ((rc = grow(h->ctl, h->ctl_cap, 2 + (size_t)arrive, "counters"))) return rc;
+  if ((rc = grow(h->ctl, h->ctl_cap, kCtlArrive + (size_t)arrive, "counters"))) return rc;
+  if ((rc = grow(h->qoff, h->qoff_cap, qoff.size(), "queue offsets"))) return rc;
   if ((rc = grow(h->epi, h->epi_cap, epi.size(), "epilogue items"))) return rc;
   hipStream_t st = iqsi::stream();
   IQSI_TRY(hipMemcpyAsync(h->inst, I.data(), sizeof(HeaInst) * I.size(), hipMemcpyHostToDevice, st));
   if (!tasks.empty())
     IQSI_TRY(hipMemcpyAsync(h->task, tasks.data(), sizeof(HeaTask) * tasks.size(), hipMemcpyHostToDevice, st));
   if (!epi.empty()) IQSI_TRY(hipMemcpyAsync(h->epi, epi.data(), sizeof(uint2) * epi.size(), hipMemcpyHostToDevice, st));
+  IQSI_TRY(hipMemcpyAsync(h->qoff, qoff.data(), sizeof(uint32_t) * qoff.size(), hipMemcpyHostToDevice, st));
   IQSI_TRY(hipStreamSynchronize(st));
+  h->qlen.assign(kQueues, 0);
+  for (int qi = 0; qi < kQueues; ++qi) h->qlen[qi] = qoff[qi + 1] - qoff[qi];
   h->ntask = (uint32_t)tasks.size();
   h->nepi = (uint32_t)epi.size();
-  h->nctl = 2 + arrive;
+  h->nctl = kCtlArrive + arrive;
   h->built_mode = mode;
   drop_graph(h);
   return IQSH_OK;
@@ -840,7 +870,7 @@ int enqueue_t(iqsh_hea* h, int mode, hipStream_t st)
   HeaArgs<T> a;
   a.inst = h->inst;
   a.task = h->task;
-  a.ntask = h->ntask;
+  a.qoff = h->qoff;
   a.flags = (uint32_t)mode;
   a.ctl = h->ctl;
   a.arena = (V<T>*)h->arena;
@@ -964,7 +994,8 @@ int iqsh_hea_destroy(iqsh_hea_handle h)
   if (!h) return IQSH_OK;
   (void)hipStreamSynchronize(iqsi::stream());
   drop_graph(h);
-  void* bufs[] = {h->arena, h->mats, h->ang, h->grad, h->cost, h->part, h->cpart, h->inst, h->task, h->ctl, h->epi};
+  void* bufs[] = {h->arena, h->mats, h->ang, h->grad, h->cost, h->part, h->cpart, h->inst, h->task, h->ctl, h->epi,
+                  h->qoff};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   if (h->cap) (void)hipStreamDestroy(h->cap);
@@ -1060,12 +1091,16 @@ int iqsh_hea_results(iqsh_hea_handle h, double* cost, double* grad)
 {
   if (int rc = check_handle(h)) return rc;
   hipStream_t st = iqsi::stream();
-  uint32_t ab = 0;
-  IQSI_TRY(hipMemcpyAsync(&ab, h->ctl + 1, sizeof ab, hipMemcpyDeviceToHost, st));
+  uint32_t ctl[kCtlArrive] = {0};
+  IQSI_TRY(hipMemcpyAsync(ctl, h->ctl, sizeof ctl, hipMemcpyDeviceToHost, st));
   if (cost) IQSI_TRY(hipMemcpyAsync(cost, h->cost, (size_t)h->count * sizeof(double), hipMemcpyDeviceToHost, st));
   if (grad) IQSI_TRY(hipMemcpyAsync(grad, h->grad, h->npairs * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
   IQSI_TRY(hipStreamSynchronize(st));
-  if (ab) return fail(IQSH_ERR_HIP, "batch run aborted: a tile dependency wait timed out");
+  if (ctl[kCtlAbort]) return fail(IQSH_ERR_HIP, "batch run aborted: a tile dependency wait timed out");
+  for (int qi = 0; qi < kQueues; ++qi)
+    if (ctl[qi] < h->qlen[qi])
+      return fail(IQSH_ERR_HIP, "batch run incomplete: queue %d served %u of %u tasks (no workgroup on its XCD)", qi,
+                  ctl[qi], h->qlen[qi]);
   return IQSH_OK;
 }
 

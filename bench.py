@@ -301,6 +301,8 @@ def main() -> int:
     ap.add_argument("--qaoa-qubits", type=int, default=28)
     ap.add_argument("--no-fusion", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the HEA batch section (for profiling its kernels); no metric value")
     ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--fusion-qubits", type=int, default=32)
     args = ap.parse_args()
@@ -326,6 +328,10 @@ def main() -> int:
     torch.cuda.init()
     stream = torch.cuda.current_stream()
     call("iqsh_set_stream", stream.cuda_stream)
+
+    if args.batch_only:
+        print(json.dumps({"hea_batch": batch_measure(args.batch_reps, torch)}))
+        return 0
 
     n = args.qubits
     S = (1 << n) * 16

@@ -34,8 +34,9 @@
 //    s-1 of its instance have arrived.  Every task waits only on tasks earlier in the
 //    queue, which were taken by running workgroups, so the queue always drains; waits are
 //    bounded and set an abort flag instead of hanging.
-//  * Gradient sums are FP64: per gate a wave sum, per tile a fixed-order sum over waves,
-//    written as per-tile partials and reduced in tile order by an epilogue kernel.
+//  * Gradient sums are FP64 (products of FP32 amplitudes are formed in FP32): per gate a DPP
+//    wave sum, per tile a fixed-order sum over waves, written as per-tile partials and
+//    reduced in tile order by an epilogue kernel.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -193,10 +194,27 @@ __device__ __forceinline__ void ud_apply(const C* u, C& a, C& b)
 // the passes 4x slower.
 __device__ __forceinline__ void acquire_tiles() { __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent"); }
 
-__device__ __forceinline__ double wave_sum(double v)
+// One DPP row operation on a double (both halves moved alike); rows outside ROWMASK read 0.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_move(double v)
 {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROWMASK, 0xf, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROWMASK, 0xf, false);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Sum over the 64 lanes of a wavefront, complete in lane 63: quad swaps, half-row and row
+// mirrors, then row broadcasts (lane 15 into rows 1 and 3, lane 31 into rows 2 and 3).
+// Fixed order, so bitwise reproducible; VALU moves instead of LDS-routed shuffles.
+__device__ __forceinline__ double wave_sum63(double v)
+{
+  v += dpp_move<0xB1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E, 0xf>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x141, 0xf>(v);  // row_half_mirror
+  v += dpp_move<0x140, 0xf>(v);  // row_mirror
+  v += dpp_move<0x142, 0xa>(v);  // row_bcast:15
+  v += dpp_move<0x143, 0xc>(v);  // row_bcast:31
   return v;
 }
 
@@ -253,7 +271,7 @@ struct Tile {
   static __device__ void gates_bwd(C* x, C* y, uint32_t rmask, const C* row, int a, const G& g, double* redw,
                                    int slot0)
   {
-    const bool lane0 = (threadIdx.x & 63) == 0;
+    const bool lane63 = (threadIdx.x & 63) == 63;
 #pragma unroll
     for (int j = 0; j < LR; ++j) {
       if ((rmask >> j) & 1u) {
@@ -264,22 +282,22 @@ struct Tile {
         for (int k = 0; k < R; ++k) {
           if ((k >> j) & 1) continue;
           const int k1 = k | (1 << j);
-          const double p0r = x[k].x, p0i = x[k].y, p1r = x[k1].x, p1i = x[k1].y;
-          const double l0r = y[k].x, l0i = y[k].y, l1r = y[k1].x, l1i = y[k1].y;
-          // conj(l0) p1, conj(l1) p0, Im conj(l0) p0, Im conj(l1) p1
-          const double c01r = l0r * p1r + l0i * p1i, c01i = l0r * p1i - l0i * p1r;
-          const double c10r = l1r * p0r + l1i * p0i, c10i = l1r * p0i - l1i * p0r;
-          const double c00i = l0r * p0i - l0i * p0r, c11i = l1r * p1i - l1i * p1r;
-          sx += c01i + c10i;
-          sy += c10r - c01r;
-          sz += c00i - c11i;
+          const C p0 = x[k], p1 = x[k1], l0 = y[k], l1 = y[k1];
+          // conj(l0) p1, conj(l1) p0, Im conj(l0) p0, Im conj(l1) p1: products in the
+          // amplitude precision, one per-pair combination per sum accumulated in FP64.
+          const T c01r = l0.x * p1.x + l0.y * p1.y, c01i = l0.x * p1.y - l0.y * p1.x;
+          const T c10r = l1.x * p0.x + l1.y * p0.y, c10i = l1.x * p0.y - l1.y * p0.x;
+          const T c00i = l0.x * p0.y - l0.y * p0.x, c11i = l1.x * p1.y - l1.y * p1.x;
+          sx += (double)(c01i + c10i);
+          sy += (double)(c10r - c01r);
+          sz += (double)(c00i - c11i);
           ud_apply(U, x[k], x[k1]);
           ud_apply(U, y[k], y[k1]);
         }
-        sx = wave_sum(sx);
-        sy = wave_sum(sy);
-        sz = wave_sum(sz);
-        if (lane0) {
+        sx = wave_sum63(sx);
+        sy = wave_sum63(sy);
+        sz = wave_sum63(sz);
+        if (lane63) {
           double* r = redw + slot0 + 3 * (a + j - g.la);
           r[0] = sx;
           r[1] = sy;
@@ -349,8 +367,8 @@ struct Tile {
       cs += (double)o * (re * re + im * im);
       if (lam_out) y[k] = x[k] * (T)o;
     }
-    cs = wave_sum(cs);
-    if ((tid & 63) == 0) redc[tid >> 6] = cs;
+    cs = wave_sum63(cs);
+    if ((tid & 63) == 63) redc[tid >> 6] = cs;
     __syncthreads();
     double tot = 0;
     if (tid == 0)
@@ -489,20 +507,37 @@ __global__ __launch_bounds__((Tile<T, M, R>::NT), 2) void k_hea(HeaArgs<T> A)
   const HeaTask* task = A.task + A.qoff[q];
   const uint32_t ntask = A.qoff[q + 1] - A.qoff[q];
   uint32_t* head = A.ctl + q;
-  // Thread 0 takes the next queue entry while the current task runs, so the atomic's round
-  // trip overlaps the work.
+  // Thread 0 takes queue entries one task ahead and stages the next task's descriptors in
+  // LDS while the current task drains its stores, so neither the queue atomic nor the
+  // descriptor loads sit between two tasks.
+  __shared__ HeaTask sh_tk;
+  __shared__ HeaInst sh_in;
   uint32_t next = 0;
-  if (tid == 0) next = atomicAdd(head, 1u);
+  auto stage = [&](uint32_t idx) {
+    sh[0] = idx;
+    if (idx < ntask) {
+      const HeaTask t = task[idx];
+      sh_tk = t;
+      sh_in = A.inst[t.inst];
+    }
+  };
+  if (tid == 0) {
+    stage(atomicAdd(head, 1u));
+    next = atomicAdd(head, 1u);
+  }
   for (;;) {
-    if (tid == 0) sh[0] = next;
     __syncthreads();
     const uint32_t i = sh[0];
     if (i >= ntask) return;
-    const HeaTask tk = task[i];
-    const HeaInst I = A.inst[tk.inst];
-    if (tid == 0) next = atomicAdd(head, 1u);
+    const HeaTask tk = sh_tk;
+    const HeaInst I = sh_in;
+    __syncthreads();  // every thread holds its copy before thread 0 stages the next task
     if (I.whole) {
       Tl::run_whole(A, I, tk.inst, lds, red, redc);
+      if (tid == 0) {
+        stage(next);
+        next = atomicAdd(head, 1u);
+      }
     } else {
       if (tid == 0) {
         uint32_t ok = 1;
@@ -523,9 +558,17 @@ __global__ __launch_bounds__((Tile<T, M, R>::NT), 2) void k_hea(HeaArgs<T> A)
         sh[1] = ok;
       }
       __syncthreads();
-      if (sh[1]) {
+      const bool ok = sh[1];
+      if (ok) {
         if (tk.step > 0) acquire_tiles();
         Tl::run_pass(A, I, tk.step, tk.tile, lds, red, redc);
+      }
+      __syncthreads();  // sh[1] read by all before it is reused
+      if (tid == 0) {
+        stage(next);
+        next = atomicAdd(head, 1u);
+      }
+      if (ok) {
         __builtin_amdgcn_s_waitcnt(0);  // this thread's state stores have completed
         __syncthreads();
         if (tid == 0)
@@ -533,7 +576,6 @@ __global__ __launch_bounds__((Tile<T, M, R>::NT), 2) void k_hea(HeaArgs<T> A)
                                  __HIP_MEMORY_SCOPE_AGENT);
       }
     }
-    __syncthreads();
   }
 }
 

@@ -190,6 +190,9 @@ class QubitRegister
   void AmplitudeWiseSum(QubitRegister<Type>& psi, Type factor = Type(1, 0));
   Type ComputeOverlap(QubitRegister<Type>& psi);
   void Print(std::string x, std::vector<std::size_t> qbits = {});
+  void ExportAmplitudes(std::string ofname);
+  void dumpbin(std::string fn);
+  void loadbin(std::string fn);
 
   // ---- random numbers ----
   RandomNumberGenerator<BaseType>* GetRngPtr() { return rng_ptr_; }

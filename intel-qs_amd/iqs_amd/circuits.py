@@ -183,6 +183,8 @@ def _parse_op(t: List[str]) -> Tuple:
         return (name, float(t[1]), k, [int(x) for x in t[3:3 + 2 * k]])
     if name == "QaoaPhase":
         return (name, float(t[1]))
+    if name in ("Print", "ExportAmplitudes", "dumpbin", "loadbin"):
+        return (name, t[1])
     if name == "TurnOnFusion":
         return (name, int(t[1]))
     if name == "GetGlobalAmplitude":

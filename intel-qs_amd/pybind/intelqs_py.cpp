@@ -185,8 +185,13 @@ void bind_register(py::module_& m, const char* name)
       .def("Print",
            [](Q& a, std::string description) {
              py::scoped_ostream_redirect stream(std::cout, py::module_::import("sys").attr("stdout"));
-             a.Print(description);
-           })
+             std::cout << "<<the output has been redirected to the terminal>>\n";
+             a.Print(description, {});
+           },
+           "Print the quantum state with an initial description.")
+      .def("ExportAmplitudes", &Q::ExportAmplitudes)
+      .def("dumpbin", &Q::dumpbin)
+      .def("loadbin", &Q::loadbin)
       .def("__repr__", [name](const Q& a) {
         return std::string("<") + name + " of " + std::to_string(a.NumQubits()) + " qubits in GPU memory>";
       });

@@ -5,10 +5,13 @@
 #include "qureg.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
+#include <string>
+#include <vector>
 
 #include "iqs_hip.h"
 
@@ -1133,17 +1136,88 @@ Type QubitRegister<Type>::ComputeOverlap(QubitRegister<Type>& psi)
   return Type((BaseType)re, (BaseType)im);
 }
 
+// State I/O.  The host mirror is synchronized once; amplitudes are listed in data order
+// with program-order labels (Permutation::data2program), as src/qureg_utils.cpp:461-563
+// (Print) and :566-657 (ExportAmplitudes) format them for a single rank.
 template <class Type>
 void QubitRegister<Type>::Print(std::string x, std::vector<std::size_t>)
 {
   const Type* s = host_();
   host_dirty_ = false;
-  printf("%s\n", x.c_str());
+  BaseType cumulative_probability = 0;
+  printf("qubit permutation: %s\n", qubit_permutation->GetMapStr().c_str());
+  printf("%s=[\n", x.c_str());
   for (std::size_t i = 0; i < global_size_; ++i) {
-    const std::size_t d = qubit_permutation->program2data_(i);
-    printf("\t%-13.8e + i %-13.8e    %% |%s>\n", (double)s[d].real(), (double)s[d].imag(),
-           Permutation::dec2bin(i, num_qubits).c_str());
+    const std::string bin = qubit_permutation->data2program(i);
+    printf("\t%-13.8lf + i * %-13.8lf   %% |%s> p=%lf\n", (double)std::real(s[i]), (double)std::imag(s[i]),
+           bin.c_str(), (double)std::norm(s[i]));
+    cumulative_probability += std::norm(s[i]);
   }
+  printf("]; %% cumulative probability = %lf\n", (double)cumulative_probability);
+}
+
+template <class Type>
+void QubitRegister<Type>::ExportAmplitudes(std::string ofname)
+{
+  const Type* s = host_();
+  host_dirty_ = false;
+  std::string str;
+  char line[4096];
+  for (std::size_t i = 0; i < global_size_; ++i) {
+    const std::string bin = qubit_permutation->data2program(i);
+    snprintf(line, sizeof line, "\t\t\"%s\" : [%-13.8lf, %-13.8lf, %lf],\n", bin.c_str(), (double)std::real(s[i]),
+             (double)std::imag(s[i]), (double)std::norm(s[i]));
+    str += line;
+  }
+  str = str.substr(0, str.size() - 2) + "\n";  // drop the last trailing comma
+  std::FILE* f = std::fopen(ofname.c_str(), "a");
+  if (!f) throw std::runtime_error("ExportAmplitudes: cannot open " + ofname);
+  std::fputs("\t\"amplitudes\" :\n\t{\n", f);
+  std::fputs(str.c_str(), f);
+  std::fputs("\t}\n", f);
+  std::fclose(f);
+}
+
+// Raw amplitudes in data order, sizeof(Type) bytes each.  The reference writes the same
+// layout through MPI-IO only (src/qureg_utils.cpp:661-689); here one process writes it.
+template <class Type>
+void QubitRegister<Type>::dumpbin(std::string fn)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  const Type* s = host_();
+  host_dirty_ = false;
+  std::FILE* f = std::fopen(fn.c_str(), "wb");
+  if (!f) throw std::runtime_error("dumpbin: cannot open " + fn);
+  const std::size_t w = std::fwrite(s, sizeof(Type), global_size_, f);
+  std::fclose(f);
+  if (w != global_size_) throw std::runtime_error("dumpbin: short write to " + fn);
+  const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  printf("Dumping state to %s took %lf sec (%lf MB/s)\n", fn.c_str(), dt,
+         (double)(sizeof(Type) * global_size_) / dt / 1e6);
+}
+
+// Inverse of dumpbin (no counterpart in the reference): the file must hold exactly 2^n
+// amplitudes of this precision, in data order for the register's current permutation.
+template <class Type>
+void QubitRegister<Type>::loadbin(std::string fn)
+{
+  std::FILE* f = std::fopen(fn.c_str(), "rb");
+  if (!f) throw std::runtime_error("loadbin: cannot open " + fn);
+  std::fseek(f, 0, SEEK_END);
+  const long bytes = std::ftell(f);
+  std::fseek(f, 0, SEEK_SET);
+  if (bytes < 0 || (std::size_t)bytes != sizeof(Type) * global_size_) {
+    std::fclose(f);
+    throw std::runtime_error("loadbin: " + fn + " does not hold " + std::to_string(global_size_) +
+                             " amplitudes of this precision");
+  }
+  std::vector<Type> buf(global_size_);
+  const std::size_t r = std::fread(buf.data(), sizeof(Type), global_size_, f);
+  std::fclose(f);
+  if (r != global_size_) throw std::runtime_error("loadbin: short read from " + fn);
+  ck(iqsh_upload(dev_(), buf.data(), 0, global_size_));
+  host_dirty_ = false;
+  host_valid_ = false;
 }
 
 template class QubitRegister<ComplexSP>;

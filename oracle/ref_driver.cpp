@@ -129,6 +129,8 @@ struct Runner {
     else if (name == "TurnOffSpecializeV2") { psi->TurnOffSpecializeV2(); }
     else if (name == "TurnOnFusion") { is >> a; psi->TurnOnFusion(a); }
     else if (name == "TurnOffFusion") { psi->TurnOffFusion(); }
+    else if (name == "Print") { std::string label; is >> label; std::fflush(res); psi->Print(label); std::fflush(stdout); }
+    else if (name == "ExportAmplitudes") { std::string f; is >> f; psi->ExportAmplitudes(f); }
     else if (name == "QaoaPhase") { is >> t; ensure_diag(); iqs::qaoa::ImplementQaoaLayerBasedOnCostFunction(*psi, *diag, BT(t)); }
     // ---- read-outs ----
     else if (name == "GetProbability") { is >> a; fprintf(res, "R GetProbability %u %.17g\n", a, (double)psi->GetProbability(a)); }

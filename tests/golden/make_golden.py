@@ -199,9 +199,45 @@ def run_reference(text: str):
     return st, res
 
 
+def case_io(prec: str) -> Circuit:
+    """State I/O: Print (stdout text) and ExportAmplitudes (JSON fragment) after a qubit
+    permutation, so labels are program-order strings of data-order amplitudes.  The
+    ExportAmplitudes argument OUT is replaced by a file path when the circuit runs."""
+    rng = SplitMix64(909)
+    c = Circuit(4, precision=prec)
+    prep(c, rng)
+    c.add("PermuteQubits", "direct", [2, 0, 3, 1])
+    c.add("ApplyHadamard", 1)
+    c.add("ApplyCPauliX", 3, 0)
+    c.add("Print", "state")
+    c.add("ExportAmplitudes", "OUT")
+    return c
+
+
+def make_io() -> None:
+    """io_<prec>.iocirc (kept apart from the state fixtures), io_print_<prec>.txt (the
+    reference's stdout) and io_export_<prec>.json."""
+    for prec in ("dp", "sp"):
+        c = case_io(prec)
+        text = c.to_text()
+        with tempfile.TemporaryDirectory() as d:
+            cf, out = os.path.join(d, "c.txt"), os.path.join(d, "export.json")
+            with open(cf, "w") as f:
+                f.write(text.replace("ExportAmplitudes OUT", f"ExportAmplitudes {out}"))
+            r = subprocess.run([DRIVER, cf, "--results", os.path.join(d, "r.txt")], check=True,
+                               env=dict(os.environ, OMP_NUM_THREADS="1"), capture_output=True, text=True)
+            export = open(out).read()
+        for suffix, data in ((".iocirc", text), ("_print.txt", r.stdout), ("_export.json", export)):
+            name = f"io_{prec}{suffix}" if suffix == ".iocirc" else f"io{suffix.replace('.', '_' + prec + '.', 1)}"
+            with open(os.path.join(HERE, name), "w") as f:
+                f.write(data)
+        print(f"io_{prec}: print {len(r.stdout.splitlines())} lines, export {len(export.splitlines())} lines")
+
+
 def main() -> None:
     if not os.path.exists(DRIVER):
         sys.exit(f"{DRIVER} missing: run `make -C oracle ref` first")
+    make_io()
     for name, fn in CASES.items():
         c = fn()
         text = c.to_text()

@@ -197,6 +197,14 @@ class QubitRegister
   // ---- random numbers ----
   RandomNumberGenerator<BaseType>* GetRngPtr() { return rng_ptr_; }
   void ResetRngPtr() { rng_ptr_ = nullptr; }
+
+  // ---- noisy simulation (src/qureg_noisysimul.cpp) ----
+  BaseType GetT1() { return T_1_; }
+  BaseType GetT2() { return T_2_; }
+  BaseType GetTphi() { return T_phi_; }
+  void SetNoiseTimescales(BaseType T1, BaseType T2);
+  void ApplyNoiseGate(const unsigned qubit, const BaseType duration);
+  BaseType GetOverallSignOfChannels() { return overall_sign_of_channels; }
   void SetRngPtr(RandomNumberGenerator<BaseType>* p) { rng_ptr_ = p; }
   void SetSeedRngPtr(std::size_t seed)
   {
@@ -247,6 +255,7 @@ class QubitRegister
   std::size_t fused_gates_ = 0, fused_blocks_ = 0;
   std::shared_ptr<std::vector<Pending>> queue_;
   RandomNumberGenerator<BaseType>* rng_ptr_ = nullptr;
+  BaseType T_1_ = 0, T_2_ = 0, T_phi_ = 0;  // noise timescales (chosen time unit)
   static bool do_print_extra_info;
 
   QubitRegister& operator=(const QubitRegister&) = delete;

@@ -44,7 +44,8 @@ class RandomNumberGenerator
   }
   void GaussianRandomNumbers(Type* out, std::size_t size, std::string shared = "local")
   {
-    std::mt19937* g = select(size, shared);
+    // Counted as two uniforms per Gaussian (src/rng_utils.cpp:196-210).
+    std::mt19937* g = select(2 * size, shared);
     for (std::size_t i = 0; i < size; ++i) out[i] = n_(*g);
   }
   template <typename IntType>

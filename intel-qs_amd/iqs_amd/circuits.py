@@ -185,6 +185,12 @@ def _parse_op(t: List[str]) -> Tuple:
         return (name, float(t[1]))
     if name in ("Print", "ExportAmplitudes", "dumpbin", "loadbin"):
         return (name, t[1])
+    if name == "SetRngSeed":
+        return (name, int(t[1]))
+    if name == "SetNoiseTimescales":
+        return (name, float(t[1]), float(t[2]))
+    if name == "ApplyNoiseGate":
+        return (name, int(t[1]), float(t[2]))
     if name == "TurnOnFusion":
         return (name, int(t[1]))
     if name == "GetGlobalAmplitude":

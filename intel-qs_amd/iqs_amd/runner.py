@@ -42,6 +42,7 @@ def run(c: Circuit, reg=None) -> Tuple[object, List[str]]:
     cdt = np.complex64 if c.precision == "sp" else np.complex128
     reg = new_register(c) if reg is None else reg
     out: List[str] = []
+    keep: List[object] = []  # objects the register points to (generators)
     diag = None
 
     def ensure_diag():
@@ -80,6 +81,12 @@ def run(c: Circuit, reg=None) -> Tuple[object, List[str]]:
             reg.CollapseQubit(args[0], bool(args[1]))
         elif name == "TurnOnFusion":
             reg.TurnOnFusion(args[0])
+        elif name == "SetRngSeed":
+            # the module binds RandomNumberGenerator for ComplexDP registers (as the reference's)
+            rng = m.RandomNumberGenerator()
+            rng.SetSeedStreamPtrs(args[0])
+            reg.SetRngPtr(rng)
+            keep.append(rng)
         elif name == "QaoaPhase":
             m.ImplementQaoaLayerBasedOnCostFunction(reg, ensure_diag(), args[0])
         elif name == "GetProbability":
@@ -101,6 +108,8 @@ def run(c: Circuit, reg=None) -> Tuple[object, List[str]]:
         else:
             getattr(reg, name)(*args)
     reg.TurnOffFusion()
+    if keep:
+        reg._iqs_keep = keep  # the register holds a raw pointer to the generator
     return reg, out
 
 

@@ -149,6 +149,11 @@ void bind_register(py::module_& m, const char* name)
       // random numbers
       .def("ResetRngPtr", &Q::ResetRngPtr)
       .def("SetRngPtr", &Q::SetRngPtr)
+      .def("GetT1", &Q::GetT1)
+      .def("GetT2", &Q::GetT2)
+      .def("GetTphi", &Q::GetTphi)
+      .def("SetNoiseTimescales", &Q::SetNoiseTimescales)
+      .def("ApplyNoiseGate", &Q::ApplyNoiseGate)
       .def("SetSeedRngPtr", &Q::SetSeedRngPtr)
       // measurement, norms, expectation values
       .def("GetProbability", &Q::GetProbability)

@@ -1220,6 +1220,51 @@ void QubitRegister<Type>::loadbin(std::string fn)
   host_valid_ = false;
 }
 
+// Noise (src/qureg_noisysimul.cpp:18-80): a stochastic Pauli-twirl rotation per call,
+// U = exp(-i vX X) exp(-i vY Y) exp(-i vZ Z) with Gaussian angles drawn from the state
+// stream of the register's generator, built and applied exactly as the reference does.
+template <class Type>
+void QubitRegister<Type>::SetNoiseTimescales(BaseType T1, BaseType T2)
+{
+  if (!(T2 >= T1 / 2.)) throw std::invalid_argument("SetNoiseTimescales: T2 must be >= T1/2");
+  T_1_ = T1;
+  T_2_ = T2;
+  T_phi_ = 1. / (1. / T2 - 1. / (2. * T1));
+}
+
+template <class Type>
+void QubitRegister<Type>::ApplyNoiseGate(const unsigned qubit, const BaseType duration)
+{
+  if (!rng_ptr_) throw std::logic_error("ApplyNoiseGate: no random number generator (SetRngPtr)");
+  if (duration == 0) return;
+  BaseType p_X, p_Y, p_Z;
+  p_X = (1. - std::exp(-duration / T_1_)) / 4.;
+  p_Y = (1. - std::exp(-duration / T_1_)) / 4.;
+  p_Z = (1. - std::exp(-duration / T_2_)) / 2. + (1. - std::exp(-duration / T_1_)) / 4.;
+  if (!(p_X > 0 && p_Y > 0 && p_Z > 0)) throw std::invalid_argument("ApplyNoiseGate: non-positive probabilities");
+  BaseType s_X, s_Y, s_Z;
+  s_X = std::sqrt(-std::log(1. - p_X));
+  s_Y = std::sqrt(-std::log(1. - p_Y));
+  s_Z = std::sqrt(-std::log(1. - p_Z));
+  BaseType v_X, v_Y, v_Z;
+  rng_ptr_->GaussianRandomNumbers(&v_X, 1, "state");
+  v_X *= s_X / 2.;
+  rng_ptr_->GaussianRandomNumbers(&v_Y, 1, "state");
+  v_Y *= s_Y / 2.;
+  rng_ptr_->GaussianRandomNumbers(&v_Z, 1, "state");
+  v_Z *= s_Z / 2.;
+  Type A, B, C;
+  A = {std::cos(v_Z), -std::sin(v_Z)};
+  B = {std::cos(v_X) * std::cos(v_Y), -std::sin(v_X) * std::sin(v_Y)};
+  C = {std::cos(v_X) * std::sin(v_Y), -std::sin(v_X) * std::cos(v_Y)};
+  TM2x2<Type> U;
+  U(0, 0) = A * B;
+  U(0, 1) = -std::conj(A) * std::conj(C);
+  U(1, 0) = A * C;
+  U(1, 1) = std::conj(A) * std::conj(B);
+  Apply1QubitGate(qubit, U);
+}
+
 template class QubitRegister<ComplexSP>;
 template class QubitRegister<ComplexDP>;
 

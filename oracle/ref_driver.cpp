@@ -37,6 +37,7 @@ struct Runner {
   Graph g;
   FILE* res = stdout;
   bool timing = false;
+  iqs::RandomNumberGenerator<BT> rng;  // attached by SetRngSeed
 
   static Type C(double re, double im) { return Type(BT(re), BT(im)); }
 
@@ -129,6 +130,13 @@ struct Runner {
     else if (name == "TurnOffSpecializeV2") { psi->TurnOffSpecializeV2(); }
     else if (name == "TurnOnFusion") { is >> a; psi->TurnOnFusion(a); }
     else if (name == "TurnOffFusion") { psi->TurnOffFusion(); }
+    else if (name == "SetRngSeed") {
+      std::size_t seed; is >> seed;
+      rng.SetSeedStreamPtrs(seed);
+      psi->SetRngPtr(&rng);
+    }
+    else if (name == "SetNoiseTimescales") { double t1, t2; is >> t1 >> t2; psi->SetNoiseTimescales(BT(t1), BT(t2)); }
+    else if (name == "ApplyNoiseGate") { double d; is >> a >> d; psi->ApplyNoiseGate(a, BT(d)); }
     else if (name == "Print") { std::string label; is >> label; std::fflush(res); psi->Print(label); std::fflush(stdout); }
     else if (name == "ExportAmplitudes") { std::string f; is >> f; psi->ExportAmplitudes(f); }
     else if (name == "QaoaPhase") { is >> t; ensure_diag(); iqs::qaoa::ImplementQaoaLayerBasedOnCostFunction(*psi, *diag, BT(t)); }

@@ -234,10 +234,41 @@ def make_io() -> None:
         print(f"io_{prec}: print {len(r.stdout.splitlines())} lines, export {len(export.splitlines())} lines")
 
 
+def case_noise() -> Circuit:
+    """Noise gates (src/qureg_noisysimul.cpp): seeded Gaussian Pauli-twirl rotations on an
+    entangled state.  The reference module binds the generator for ComplexDP only."""
+    rng = SplitMix64(515)
+    c = Circuit(6, precision="dp")
+    prep(c, rng)
+    c.add("SetRngSeed", 777)
+    c.add("SetNoiseTimescales", 50.0, 30.0)
+    for k in range(18):
+        c.add("ApplyNoiseGate", k % 6, [0.5, 2.0, 7.5][k % 3])
+        if k % 5 == 4:
+            c.add("ApplyCPauliX", k % 6, (k + 1) % 6)
+    readout(c)
+    return c
+
+
+def make_noise() -> None:
+    """noise_dp.noisecirc, .noiseres (read-outs) and noise_dp_state.npz (kept apart from the
+    oracle's fixture set: the oracle restates no random number generator)."""
+    c = case_noise()
+    text = c.to_text()
+    st, res = run_reference(text)
+    with open(os.path.join(HERE, "noise_dp.noisecirc"), "w") as f:
+        f.write(text)
+    with open(os.path.join(HERE, "noise_dp.noiseres"), "w") as f:
+        f.write(res)
+    np.savez_compressed(os.path.join(HERE, "noise_dp_state.npz"), state=st)
+    print(f"noise_dp: ops={len(c.ops)} results={len(res.splitlines())}")
+
+
 def main() -> None:
     if not os.path.exists(DRIVER):
         sys.exit(f"{DRIVER} missing: run `make -C oracle ref` first")
     make_io()
+    make_noise()
     for name, fn in CASES.items():
         c = fn()
         text = c.to_text()

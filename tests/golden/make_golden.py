@@ -264,11 +264,38 @@ def make_noise() -> None:
     print(f"noise_dp: ops={len(c.ops)} results={len(res.splitlines())}")
 
 
+QASM_REF = os.path.join(ROOT, "oracle", "_ref", "iqs_qasm_ref")
+QASM_SCRIPTS = {
+    # every operation of the interface, qubit ids assigned by first use (the scaffold
+    # "qubit"/"cbit" lines and PrepZ assign none), failing and unknown operations
+    "qasm_mixed": [".iversion", ".version", ".malloc 5", "qubit q0", "cbit c0", "H q0", "CNOT q0,q1", "T q1",
+                   "Tdag q2", "H q2", "S q3", "X q4", "H q3", "CNOT q3,q4", "PrepZ q0", "MeasZ q0", "MeasZ q1",
+                   "MeasZ q2", "MeasZ q3", "MeasZ q4", "Foo bar", "*", ".malloc 3", ".free", ".free",
+                   ".malloc 100", ".malloc abc"],
+    # GHZ-like circuit on named qubits with T/Tdag phases
+    "qasm_ghz": [".malloc 6"] + [f"H r{i}" for i in range(6)] + [f"CNOT r{i},r{i + 1}" for i in range(5)]
+                + ["T r0", "Tdag r5", "H r0", "H r5", "S r2", "H r2"] + [f"MeasZ r{i}" for i in range(6)] + [".free"],
+}
+
+
+def make_qasm() -> None:
+    """qasm_<name>.qasm and the reference program's stdout/stderr (.out/.err)."""
+    for name, lines in QASM_SCRIPTS.items():
+        text = "\n".join(lines) + "\n"
+        r = subprocess.run([QASM_REF], input=text, capture_output=True, text=True, check=True,
+                           env=dict(os.environ, OMP_NUM_THREADS="1"))
+        for suffix, data in ((".qasm", text), (".out", r.stdout), (".err", r.stderr)):
+            with open(os.path.join(HERE, name + suffix), "w") as f:
+                f.write(data)
+        print(f"{name}: {len(lines)} lines, stdout {len(r.stdout.splitlines())} lines, stderr {len(r.stderr.splitlines())}")
+
+
 def main() -> None:
     if not os.path.exists(DRIVER):
         sys.exit(f"{DRIVER} missing: run `make -C oracle ref` first")
     make_io()
     make_noise()
+    make_qasm()
     for name, fn in CASES.items():
         c = fn()
         text = c.to_text()

@@ -338,6 +338,123 @@ __global__ __launch_bounds__(B) void k_quad(typename Vec<T>::t* __restrict__ st,
   }
 }
 
+// Dense 4x4 (or swap-like move) on positions pa (row bit 1) and pb (row bit 0) when the
+// lower one is below the wavefront width: every load and store is a contiguous run of the
+// state and partners inside a wavefront come through __shfl_xor, instead of 16-byte
+// accesses spread over the cache lines.  lo = min(pa, pb) < 6.
+//  * hi = max(pa, pb) >= 6: items enumerate indices with bit hi clear; a thread holds the
+//    amplitudes with bit hi = 0 and 1 and exchanges both with lane ^ (1 << lo).
+//  * hi < 6: a thread holds one amplitude and receives the other three by shuffles.
+// MOVE: exact exchange of rows 1 and 2 (SWAP-type), no arithmetic.  Requires the item count
+// (N/2 or N) to be a multiple of B*P and >= kTiledMin.
+template <typename T, typename V>
+__device__ __forceinline__ V quad_row(const Mat4<T>& m, int rw, const V& i0, const V& i1, const V& i2, const V& i3)
+{
+  V r = cmad2(m.m[4 * rw + 0], i0, m.m[4 * rw + 1], i1);
+  const V s = cmad2(m.m[4 * rw + 2], i2, m.m[4 * rw + 3], i3);
+  r.x += s.x;
+  r.y += s.y;
+  return r;
+}
+
+// LOB: the lower position is pb (row bit 0); otherwise pa.  Inputs are named by (bit lo,
+// bit hi); rows are row(l, h) = LOB ? 2h + l : 2l + h, all compile-time.
+template <typename T, bool MOVE, bool LOB, int B, int P>
+__device__ __forceinline__ void quad_lane_body(typename Vec<T>::t* __restrict__ st, unsigned lo, unsigned hi,
+                                               const Mat4<T>& m)
+{
+  using V = typename Vec<T>::t;
+  constexpr int R00 = 0, R10 = LOB ? 1 : 2, R01 = LOB ? 2 : 1, R11 = 3;  // row(l, h)
+  const uint64_t r0 = xcd_block(block_id(), grid_blocks()) * (uint64_t)(B * P) + threadIdx.x;
+  const int ml = 1 << lo;
+  const bool ul = (threadIdx.x >> lo) & 1;  // this lane's bit lo
+  auto rowv = [&](int rw, const V& x00, const V& x10, const V& x01, const V& x11) {
+    // input order by row index
+    const V& i1 = LOB ? x10 : x01;
+    const V& i2 = LOB ? x01 : x10;
+    return quad_row(m, rw, x00, i1, i2, x11);
+  };
+  if (hi >= 6) {
+    const uint64_t hb = uint64_t(1) << hi;
+    V a[P], b[P];
+    uint64_t idx[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      idx[j] = ins0(r0 + (uint64_t)j * B, hi);
+      a[j] = ldg(&st[idx[j]]);
+      b[j] = ldg(&st[idx[j] | hb]);
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      V pa0, pb1;  // the partner lane's amplitudes with bit hi = 0 and 1
+      pa0.x = __shfl_xor(a[j].x, ml);
+      pa0.y = __shfl_xor(a[j].y, ml);
+      pb1.x = __shfl_xor(b[j].x, ml);
+      pb1.y = __shfl_xor(b[j].y, ml);
+      V o0, o1;
+      if (MOVE) {
+        // rows 1 and 2 are (l, h) = (1, 0) and (0, 1): exchange across lanes and halves.
+        o0 = ul ? pb1 : a[j];
+        o1 = ul ? b[j] : pa0;
+      } else {
+        const V x00 = ul ? pa0 : a[j], x10 = ul ? a[j] : pa0;
+        const V x01 = ul ? pb1 : b[j], x11 = ul ? b[j] : pb1;
+        const V y00 = rowv(R00, x00, x10, x01, x11), y10 = rowv(R10, x00, x10, x01, x11);
+        const V y01 = rowv(R01, x00, x10, x01, x11), y11 = rowv(R11, x00, x10, x01, x11);
+        o0 = ul ? y10 : y00;
+        o1 = ul ? y11 : y01;
+      }
+      stg(o0, &st[idx[j]]);
+      stg(o1, &st[idx[j] | hb]);
+    }
+  } else {
+    const int mh = 1 << hi;
+    const bool uh = (threadIdx.x >> hi) & 1;
+    V v[P];
+    uint64_t idx[P];
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      idx[j] = r0 + (uint64_t)j * B;
+      v[j] = ldg(&st[idx[j]]);
+    }
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      V xl, xh, xb;
+      xl.x = __shfl_xor(v[j].x, ml);
+      xl.y = __shfl_xor(v[j].y, ml);
+      xh.x = __shfl_xor(v[j].x, mh);
+      xh.y = __shfl_xor(v[j].y, mh);
+      xb.x = __shfl_xor(v[j].x, ml | mh);
+      xb.y = __shfl_xor(v[j].y, ml | mh);
+      V o;
+      if (MOVE) {
+        o = (ul != uh) ? xb : v[j];  // (1,0) <-> (0,1)
+      } else {
+        // (l, h) of this lane's amplitude is (ul, uh); name the four by absolute bits.
+        const V x00 = ul ? (uh ? xb : xl) : (uh ? xh : v[j]);
+        const V x10 = ul ? (uh ? xh : v[j]) : (uh ? xb : xl);
+        const V x01 = ul ? (uh ? xl : xb) : (uh ? v[j] : xh);
+        const V x11 = ul ? (uh ? v[j] : xh) : (uh ? xl : xb);
+        const V y00 = rowv(R00, x00, x10, x01, x11), y10 = rowv(R10, x00, x10, x01, x11);
+        const V y01 = rowv(R01, x00, x10, x01, x11), y11 = rowv(R11, x00, x10, x01, x11);
+        o = ul ? (uh ? y11 : y10) : (uh ? y01 : y00);
+      }
+      stg(o, &st[idx[j]]);
+    }
+  }
+}
+
+template <typename T, bool MOVE, int B, int P>
+__global__ __launch_bounds__(B) void k_quad_lane(typename Vec<T>::t* __restrict__ st, unsigned pa, unsigned pb,
+                                                 Mat4<T> m)
+{
+  const unsigned lo = pa < pb ? pa : pb, hi = pa < pb ? pb : pa;
+  if (lo == pb)
+    quad_lane_body<T, MOVE, true, B, P>(st, lo, hi, m);
+  else
+    quad_lane_body<T, MOVE, false, B, P>(st, lo, hi, m);
+}
+
 // ---------------------------------------------------------------------------------------
 // Dense 2^KQ block on arbitrary positions.  A block stages G groups of 2^KQ amplitudes in
 // LDS (gathered through the position masks), then each thread produces one output

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
-from iqs_amd.circuits import SplitMix64, haar_unitary
+from iqs_amd.circuits import SplitMix64, haar_unitary, random_su2
 from iqs_amd.engine import StateVector
 
 pytestmark = pytest.mark.gpu
@@ -454,3 +454,47 @@ def test_block34_padded_to_mfma(dt, positions):
     t = 1e-12 if dt == "c128" else 3e-5
     assert np.abs(got - want).max() < t
     assert np.abs(valu - want).max() < t
+
+
+# ---- registers of the tiled size (n >= 17): lane-shuffle 4x4 and line-aligned controls ----
+
+@pytest.mark.parametrize("ph,pl", [(0, 1), (1, 0), (2, 16), (16, 0), (5, 3), (3, 5), (7, 4), (6, 5), (12, 9)])
+def test_2q_dense_tiled_low_positions(dt, ph, pl):
+    n = 17
+    m = haar_unitary(4, SplitMix64(31 * ph + pl))
+    h = rand_state(n, dt, ph + 7 * pl)
+    sv = on_gpu(h, dt)
+    sv.apply_2q(ph, pl, m)
+    oracle.apply_2q(h, ph, pl, m)
+    assert np.abs(sv.download() - h).max() < TOL[dt]
+
+
+@pytest.mark.parametrize("p1,p2", [(0, 16), (16, 0), (1, 2), (2, 1), (0, 5), (4, 9), (3, 0)])
+def test_swaplike_tiled_low_positions(dt, p1, p2):
+    n = 17
+    h = rand_state(n, dt, 11 + p1)
+    sv = on_gpu(h, dt)
+    th = 0.41 + p2
+    m = np.array([[math.cos(th), 1j * math.sin(th)], [1j * math.sin(th), math.cos(th)]])
+    sv.apply_swaplike(p1, p2, m)
+    oracle.apply_swaplike(h, min(p1, p2), max(p1, p2), m)
+    assert np.abs(sv.download() - h).max() < TOL[dt]
+    sv.upload(h)  # identical starting bits for the exact-move check
+    sv.apply_swaplike(p1, p2, [[0, 1], [1, 0]])  # exact move
+    oracle.apply_swaplike(h, min(p1, p2), max(p1, p2), np.array([[0, 1], [1, 0]]))
+    assert np.array_equal(sv.download(), h)
+
+
+@pytest.mark.parametrize("c,t", [(3, 2), (4, 0), (5, 1), (5, 4), (3, 5), (4, 3)])
+def test_controlled_tiled_line_aligned_control(dt, c, t):
+    n = 17
+    h = rand_state(n, dt, 3 * c + t)
+    sv = on_gpu(h, dt)
+    m = random_su2(SplitMix64(c * 100 + t))
+    sv.apply_c1q(c, t, m)
+    oracle.apply_c1q(h, c, t, m)
+    assert np.abs(sv.download() - h).max() < TOL[dt]
+    sv.upload(h)  # identical starting bits for the exact-move check
+    sv.apply_c1q(c, t, [[0, 1], [1, 0]])  # CNOT: exact move
+    oracle.apply_c1q(h, c, t, np.array([[0, 1], [1, 0]]))
+    assert np.array_equal(sv.download(), h)

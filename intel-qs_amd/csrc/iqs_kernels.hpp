@@ -91,12 +91,12 @@ __device__ __forceinline__ uint64_t block_id() { return blockIdx.x + (uint64_t)b
 __device__ __forceinline__ uint64_t grid_blocks() { return (uint64_t)gridDim.x * gridDim.y; }
 
 // Item index of (block bb, slot t in [0, B*P)) under the split-block tiling.
-template <int B, int P, int SUB>
+template <int B, int P, int SUB, unsigned SL = kSubLog>
 __device__ __forceinline__ uint64_t tiled_item(uint64_t bb, unsigned t)
 {
   if (SUB == 1) return bb * (uint64_t)(B * P) + t;
   constexpr uint64_t sublen = (uint64_t)B * P / SUB;
-  constexpr uint64_t span = uint64_t(1) << kSubLog;
+  constexpr uint64_t span = uint64_t(1) << SL;
   constexpr uint64_t bps = span / sublen;
   static_assert(span % sublen == 0, "sub-range must divide the span");
   const uint64_t win = bb / bps, off = bb % bps;
@@ -131,7 +131,7 @@ __device__ __forceinline__ V cmul(const Cx<T>& a, const V& x)
 // MOVE=true exchanges the two amplitudes (X-type matrices: a pure, bit-exact move).
 // TILED=true: XCD-contiguous, split-block tiling (nitems a power of two >= kTiledMin, grid
 // exactly nitems / (B*P)); TILED=false: plain contiguous blocks with bounds checks.
-template <typename T, int K, bool MOVE, bool TILED, int B, int P>
+template <typename T, int K, bool MOVE, bool TILED, int B, int P, int SUB = kSub, unsigned SL = kSubLog>
 __global__ __launch_bounds__(B) void k_pair(typename Vec<T>::t* __restrict__ st, uint64_t nitems, Ins<K> ins,
                                             uint64_t mask0, uint64_t mask1, Mat2<T> m)
 {
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(B) void k_pair(typename Vec<T>::t* __restrict__ st,
     const uint64_t bb = xcd_block(block_id(), grid_blocks());
 #pragma unroll
     for (int j = 0; j < P; ++j) {
-      base[j] = item_base(tiled_item<B, P, kSub>(bb, j * B + threadIdx.x), ins);
+      base[j] = item_base(tiled_item<B, P, SUB, SL>(bb, j * B + threadIdx.x), ins);
       a[j] = ldg(&st[base[j] | mask0]);
       b[j] = ldg(&st[base[j] | mask1]);
     }
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(B) void k_diag2(typename Vec<T>::t* __restrict__ st
 }
 
 // Dense 4x4 on (ph, pl): indices b, b|dl, b|dh, b|dh|dl; new[t] = sum_c m[t][c] old[c].
-template <typename T, bool TILED, int B, int P>
+template <typename T, bool TILED, int B, int P, int SUB = kSub, unsigned SL = kSubLog>
 __global__ __launch_bounds__(B) void k_quad(typename Vec<T>::t* __restrict__ st, uint64_t nitems, Ins<2> ins,
                                             uint64_t dl, uint64_t dh, Mat4<T> m)
 {
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(B) void k_quad(typename Vec<T>::t* __restrict__ st,
   bool ok[P];
 #pragma unroll
   for (int j = 0; j < P; ++j) {
-    const uint64_t k = TILED ? tiled_item<B, P, kSub>(bb, j * B + threadIdx.x) : bb * (uint64_t)(B * P) + (uint64_t)j * B + threadIdx.x;
+    const uint64_t k = TILED ? tiled_item<B, P, SUB, SL>(bb, j * B + threadIdx.x) : bb * (uint64_t)(B * P) + (uint64_t)j * B + threadIdx.x;
     ok[j] = TILED || k < nitems;
     base[j] = item_base(k, ins);
     if (ok[j]) {

@@ -19,6 +19,7 @@
 #include <tuple>
 #include <vector>
 
+#include "chi_matrix.hpp"
 #include "gate_counter.hpp"
 #include "gate_spec.hpp"
 #include "mpi_env.hpp"
@@ -32,6 +33,10 @@ template <class Type>
 using TM2x2 = iqs::TinyMatrix<Type, 2, 2, 32>;
 template <class Type>
 using TM4x4 = iqs::TinyMatrix<Type, 4, 4, 32>;
+template <class Type>
+using CM4x4 = iqs::ChiMatrix<Type, 4, 32>;
+template <class Type>
+using CM16x16 = iqs::ChiMatrix<Type, 16, 32>;
 
 struct iqsh_state;
 
@@ -204,6 +209,9 @@ class QubitRegister
   BaseType GetTphi() { return T_phi_; }
   void SetNoiseTimescales(BaseType T1, BaseType T2);
   void ApplyNoiseGate(const unsigned qubit, const BaseType duration);
+  // ---- quantum channels (src/qureg_apply_channel.cpp) ----
+  void ApplyChannel(const unsigned qubit, CM4x4<Type>& chi);
+  void ApplyChannel(const unsigned qubit1, const unsigned qubit2, CM16x16<Type>& chi);
   BaseType GetOverallSignOfChannels() { return overall_sign_of_channels; }
   void SetRngPtr(RandomNumberGenerator<BaseType>* p) { rng_ptr_ = p; }
   void SetSeedRngPtr(std::size_t seed)
@@ -268,3 +276,7 @@ template <typename Type>
 using BaseType = typename QubitRegister<Type>::BaseType;
 
 }  // namespace iqs
+
+// Derived register that inserts noise gates automatically (the reference includes it here
+// too, include/qureg.hpp:432-433).
+#include "NoisyQureg.hpp"

@@ -51,6 +51,70 @@ TM4x4<T> tm4(py::array_t<T, py::array::c_style | py::array::forcecast> a)
   return m;
 }
 
+// ChiMatrix<ComplexDP, M> as CM4x4 / CM16x16 (pybind11/intelqs_py.cpp:94-163 of the
+// reference): element access by (row, column) pairs, SolveEigenSystem, Print.  Additions: a
+// constructor from an M x M array and read access to the eigensystem.
+template <unsigned M>
+void bind_chi(py::module_& m, const char* name, const char* repr)
+{
+  using X = iqs::ChiMatrix<ComplexDP, M, 32>;
+  py::class_<X>(m, name)
+      .def(py::init<>())
+      .def(py::init([](py::array_t<ComplexDP, py::array::c_style | py::array::forcecast> a) {
+        auto b = a.request();
+        if (b.ndim != 2 || b.shape[0] != M || b.shape[1] != M)
+          throw std::runtime_error("The shape of the chi-matrix is not " + std::to_string(M) + "x" + std::to_string(M) + ".");
+        const ComplexDP* p = static_cast<const ComplexDP*>(b.ptr);
+        X x;
+        for (unsigned i = 0; i < M; ++i)
+          for (unsigned j = 0; j < M; ++j) x(i, j) = p[M * i + j];
+        return x;
+      }))
+      .def("__getitem__",
+           [](const X& a, std::pair<py::ssize_t, py::ssize_t> i) {
+             if (i.first < 0 || i.first >= M || i.second < 0 || i.second >= M) throw py::index_error();
+             return a(i.first, i.second);
+           },
+           py::is_operator())
+      .def("__setitem__",
+           [](X& a, std::pair<py::ssize_t, py::ssize_t> i, ComplexDP value) {
+             if (i.first < 0 || i.first >= M || i.second < 0 || i.second >= M) throw py::index_error();
+             a(i.first, i.second) = value;
+           },
+           py::is_operator())
+      .def("SolveEigenSystem", &X::SolveEigenSystem)
+      .def("EigensystemOfIdealHadamardChannel", &X::EigensystemOfIdealHadamardChannel)
+      .def("Print",
+           [](X& a, bool with_eigensystem) {
+             py::scoped_ostream_redirect stream(std::cout, py::module_::import("sys").attr("stdout"));
+             a.Print(with_eigensystem);
+           },
+           py::arg("with_eigensystem") = true)
+      .def("GetEigenValues", &X::GetEigenValues)
+      .def("GetEigenProbabilities", &X::GetEigenProbabilities)
+      .def("GetEigenCumulativeProbabilities", &X::GetEigenCumulativeProbabilities)
+      .def("GetEigenVectors", &X::GetEigenVectors)
+      .def("__repr__", [repr](const X&) { return std::string(repr); });
+}
+
+template <class B>
+void bind_rng(py::module_& m, const char* name)
+{
+  using R = iqs::RandomNumberGenerator<B>;
+  py::class_<R>(m, name)
+      .def(py::init<>())
+      .def("GetSeed", &R::GetSeed)
+      .def("SetSeedStreamPtrs", &R::SetSeedStreamPtrs)
+      .def("SkipeAhead", &R::SkipAhead)
+      .def("GetUniformRandomNumbers",
+           [](R& r, std::size_t size, B a, B b, std::string shared) {
+             std::vector<B> v(size);
+             r.UniformRandomNumbers(v.data(), size, a, b, shared);
+             return v;
+           })
+      .def("__repr__", [name](const R&) { return std::string("<") + name + ">"; });
+}
+
 template <class T>
 void bind_register(py::module_& m, const char* name)
 {
@@ -154,6 +218,19 @@ void bind_register(py::module_& m, const char* name)
       .def("GetTphi", &Q::GetTphi)
       .def("SetNoiseTimescales", &Q::SetNoiseTimescales)
       .def("ApplyNoiseGate", &Q::ApplyNoiseGate)
+      .def("GetOverallSignOfChannels", &Q::GetOverallSignOfChannels)
+      .def("ApplyChannel",
+           [](Q& a, unsigned qubit, const iqs::ChiMatrix<ComplexDP, 4, 32>& chi) {
+             CM4x4<T> c(chi);
+             a.ApplyChannel(qubit, c);
+           },
+           "Apply 1-qubit channel provided via its chi-matrix.")
+      .def("ApplyChannel",
+           [](Q& a, unsigned qubit1, unsigned qubit2, const iqs::ChiMatrix<ComplexDP, 16, 32>& chi) {
+             CM16x16<T> c(chi);
+             a.ApplyChannel(qubit1, qubit2, c);
+           },
+           "Apply 2-qubit channel provided via its chi-matrix.")
       .def("SetSeedRngPtr", &Q::SetSeedRngPtr)
       // measurement, norms, expectation values
       .def("GetProbability", &Q::GetProbability)
@@ -212,21 +289,49 @@ PYBIND11_MODULE(intelqs_py, m)
   m.def("EnvFinalize", []() { Environment::Finalize(); });
   m.def("EnvFinalizeDummyRanks", []() {});
 
-  py::class_<iqs::RandomNumberGenerator<double>>(m, "RandomNumberGenerator")
-      .def(py::init<>())
-      .def("GetSeed", &iqs::RandomNumberGenerator<double>::GetSeed)
-      .def("SetSeedStreamPtrs", &iqs::RandomNumberGenerator<double>::SetSeedStreamPtrs)
-      .def("SkipeAhead", &iqs::RandomNumberGenerator<double>::SkipAhead)
-      .def("GetUniformRandomNumbers",
-           [](iqs::RandomNumberGenerator<double>& r, std::size_t size, double a, double b, std::string shared) {
-             std::vector<double> v(size);
-             r.UniformRandomNumbers(v.data(), size, a, b, shared);
-             return v;
-           })
-      .def("__repr__", []() { return "<RandomNumberGenerator>"; });
+  bind_rng<double>(m, "RandomNumberGenerator");
+  bind_rng<float>(m, "RandomNumberGeneratorSP");  // extension: the generator QubitRegisterSP takes
+
+  bind_chi<4>(m, "CM4x4", "<ChiMatrix for 1-qubit channel>");
+  bind_chi<16>(m, "CM16x16", "<ChiMatrix for 2-qubit channel>");
 
   bind_register<ComplexDP>(m, "QubitRegister");
   bind_register<ComplexSP>(m, "QubitRegisterSP");
+
+  // Extension: the C++ NoisyQureg (include/NoisyQureg.hpp) for Python callers.
+  using NQ = iqs::NoisyQureg<ComplexDP>;
+  py::class_<NQ, QubitRegister<ComplexDP>>(m, "NoisyQureg", py::buffer_protocol(), py::dynamic_attr())
+      .def(py::init<unsigned, unsigned, double, double>(), py::arg("num_qubits"), py::arg("RNG_seed") = 12345,
+           py::arg("T1") = 2000., py::arg("T2") = 1000.)
+      .def("Initialize", &NQ::Initialize)
+      .def("ResetTimeForAllQubits", &NQ::ResetTimeForAllQubits)
+      .def("ApplyNoiseGatesOnAllQubits", &NQ::ApplyNoiseGatesOnAllQubits)
+      .def("SetDecoherenceTime", &NQ::SetDecoherenceTime)
+      .def("SetGateDurations", &NQ::SetGateDurations)
+      .def("GetTotalExperimentalGateCount", &NQ::GetTotalExperimentalGateCount)
+      .def("GetOneQubitExperimentalGateCount", &NQ::GetOneQubitExperimentalGateCount)
+      .def("GetTwoQubitExperimentalGateCount", &NQ::GetTwoQubitExperimentalGateCount)
+      .def("GetExperimentalGateCount", py::overload_cast<unsigned>(&NQ::GetExperimentalGateCount))
+      .def("GetExperimentalGateCount", py::overload_cast<unsigned, unsigned>(&NQ::GetExperimentalGateCount))
+      .def("AddNoiseOneQubitGate", &NQ::AddNoiseOneQubitGate)
+      .def("AddNoiseTwoQubitGate", &NQ::AddNoiseTwoQubitGate)
+      .def("NoiseGate", &NQ::NoiseGate)
+      .def("Apply1QubitGate",
+           [](NQ& a, unsigned q, py::array_t<ComplexDP, py::array::c_style | py::array::forcecast> m) {
+             a.Apply1QubitGate(q, tm2<ComplexDP>(m, "Input shape is not 2x2."));
+           })
+      .def("ApplyHadamard", &NQ::ApplyHadamard)
+      .def("ApplyRotationX", &NQ::ApplyRotationX)
+      .def("ApplyRotationY", &NQ::ApplyRotationY)
+      .def("ApplyRotationZ", &NQ::ApplyRotationZ)
+      .def("ApplyCPauliX", &NQ::ApplyCPauliX)
+      .def("ApplyControlled1QubitGate",
+           [](NQ& a, unsigned q1, unsigned q2, py::array_t<ComplexDP, py::array::c_style | py::array::forcecast> m) {
+             a.ApplyControlled1QubitGate(q1, q2, tm2<ComplexDP>(m, "Input shape is not 2x2."));
+           })
+      .def("__repr__", [](const NQ& a) {
+        return "<NoisyQureg of " + std::to_string(a.NumQubits()) + " qubits in GPU memory>";
+      });
 
   m.def("InitializeVectorAsMaxCutCostFunction", &iqs::qaoa::InitializeVectorAsMaxCutCostFunction<ComplexDP>,
         "Use IQS vector to store a large real vector and not as a quantum state.");

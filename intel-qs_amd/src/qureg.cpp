@@ -1265,6 +1265,81 @@ void QubitRegister<Type>::ApplyNoiseGate(const unsigned qubit, const BaseType du
   Apply1QubitGate(qubit, U);
 }
 
+// Branch index for a uniform draw r in [0, 1): the first k with r <= cumulative p_k
+// (qureg_apply_channel.cpp:23-31).  The cumulative distribution ends at exactly 1 unless
+// chi is zero, which is reported.
+template <class Chi>
+static unsigned channel_branch(const Chi& chi, double r, unsigned M)
+{
+  if (!chi.HasEigenSystem()) throw std::logic_error("ApplyChannel: call SolveEigenSystem() on the chi-matrix first");
+  unsigned k = 0;
+  while (r > chi.GetEigenCumulativeProbability(k))
+    if (++k >= M) throw std::runtime_error("ApplyChannel: the chi-matrix has no non-zero eigenvalue");
+  return k;
+}
+
+/// One stochastic step of a 1-qubit channel (qureg_apply_channel.cpp:17-58): draw a branch
+/// k with probability |E_k| / sum |E_k| from the register's state stream, apply
+/// E_k = e0 id + e1 X + e2 Y + e3 Z as a 1-qubit gate, and flip the overall sign of
+/// channels for a negative eigenvalue.
+template <class Type>
+void QubitRegister<Type>::ApplyChannel(const unsigned qubit, CM4x4<Type>& chi)
+{
+  if (!rng_ptr_) throw std::logic_error("ApplyChannel: no random number generator (SetRngPtr)");
+  BaseType r;
+  rng_ptr_->UniformRandomNumbers(&r, 1, 0, 1, "state");
+  const unsigned k = channel_branch(chi, r, 4);
+  const std::vector<Type> e = chi.GetEigenVector(k);
+  const Type I(0., 1.);
+  TM2x2<Type> m;
+  m(0, 0) = e[0] + e[3];
+  m(0, 1) = e[1] - I * e[2];
+  m(1, 0) = e[1] + I * e[2];
+  m(1, 1) = e[0] - e[3];
+  if (std::real(chi.GetEigenValue(k)) < 0) overall_sign_of_channels *= -1;
+  Apply1QubitGate(qubit, m);
+}
+
+/// One stochastic step of a 2-qubit channel (qureg_apply_channel.cpp:66-106): the branch
+/// matrix is sum_{a,b} e_{4a+b} sigma_a (qubit1) x sigma_b (qubit2) in the 4x4 index
+/// 2 b(qubit1) + b(qubit2), applied with Apply2QubitGate(qubit1, qubit2, .).  The diagonal
+/// and upper triangle use the reference's expressions; the lower triangle is evaluated from
+/// the same sum (the reference writes it as the conjugate of the upper one, which equals
+/// the sum only for real coefficients) and, as in the 1-qubit case, a negative eigenvalue
+/// flips the overall sign of channels.
+template <class Type>
+void QubitRegister<Type>::ApplyChannel(const unsigned qubit1, const unsigned qubit2, CM16x16<Type>& chi)
+{
+  if (!rng_ptr_) throw std::logic_error("ApplyChannel: no random number generator (SetRngPtr)");
+  BaseType r;
+  rng_ptr_->UniformRandomNumbers(&r, 1, 0, 1, "state");
+  const unsigned k = channel_branch(chi, r, 16);
+  const std::vector<Type> e = chi.GetEigenVector(k);
+  const Type I(0., 1.);
+  TM4x4<Type> m;
+  m(0, 0) = e[0] + e[3] + e[12] + e[15];
+  m(1, 1) = e[0] - e[3] + e[12] - e[15];
+  m(2, 2) = e[0] + e[3] - e[12] - e[15];
+  m(3, 3) = e[0] - e[3] - e[12] + e[15];
+
+  m(0, 1) = e[1] + e[13] - I * (e[2] + e[14]);
+  m(1, 0) = e[1] + e[13] + I * (e[2] + e[14]);
+  m(2, 3) = e[1] - e[13] - I * (e[2] - e[14]);
+  m(3, 2) = e[1] - e[13] + I * (e[2] - e[14]);
+
+  m(0, 2) = e[4] + e[7] - I * (e[8] + e[11]);
+  m(2, 0) = e[4] + e[7] + I * (e[8] + e[11]);
+  m(1, 3) = e[4] - e[7] - I * (e[8] - e[11]);
+  m(3, 1) = e[4] - e[7] + I * (e[8] - e[11]);
+
+  m(0, 3) = e[5] - e[10] - I * (e[9] + e[6]);
+  m(3, 0) = e[5] - e[10] + I * (e[9] + e[6]);
+  m(1, 2) = e[5] + e[10] - I * (e[9] - e[6]);
+  m(2, 1) = e[5] + e[10] + I * (e[9] - e[6]);
+  if (std::real(chi.GetEigenValue(k)) < 0) overall_sign_of_channels *= -1;
+  Apply2QubitGate(qubit1, qubit2, m);
+}
+
 template class QubitRegister<ComplexSP>;
 template class QubitRegister<ComplexDP>;
 

@@ -290,11 +290,54 @@ def make_qasm() -> None:
         print(f"{name}: {len(lines)} lines, stdout {len(r.stdout.splitlines())} lines, stderr {len(r.stderr.splitlines())}")
 
 
+NOISY_REF = os.path.join(ROOT, "oracle", "_ref", "noisy_driver")
+
+
+def noisy_program() -> str:
+    """Two noisy trajectories of one 6-qubit circuit on iqs::NoisyQureg (the pattern of the
+    reference's examples/circuit_with_noise_gates.cpp): logical 1-qubit gates of every kind,
+    CNOTs and controlled gates on qubit pairs, final idle noise, gate counts; the second
+    trajectory re-initializes and continues the same generator."""
+    rs = np.random.default_rng(20240610)
+    q, _ = np.linalg.qr(rs.normal(size=(2, 2)) + 1j * rs.normal(size=(2, 2)))
+    g = " ".join(f"{x:.17g}" for v in q.reshape(-1) for x in (v.real, v.imag))
+    lines = ["NoisyQureg 6 4321 40 25", "SetGateDurations 1 3"]
+    for traj in range(2):
+        lines += ["Initialize base 0", "ResetTimeForAllQubits"]
+        for i, (a, b) in enumerate([(0, 1), (2, 3), (4, 5), (1, 2), (3, 4), (5, 0)]):
+            lines.append(f"ApplyCPauliX {a} {b}" if i % 2 else f"ApplyControlled1QubitGate {a} {b} {g}")
+        for k in range(6):
+            lines.append(f"Apply1QubitGate {k} {g}")
+        lines += ["ApplyHadamard 2", "ApplyRotationX 0 0.37", "ApplyRotationY 4 -1.1", "ApplyRotationZ 5 2.2",
+                  "ApplyNoiseGatesOnAllQubits", "Counts"]
+        if traj == 0:
+            lines.append("SetDecoherenceTime 60 35")
+    return "\n".join(lines) + "\n"
+
+
+def make_noisy() -> None:
+    """noisy_dp.nqprog, the reference's read-outs (.nqres) and final state (noisy_dp_state.npz)."""
+    text = noisy_program()
+    prog = os.path.join(HERE, "noisy_dp.nqprog")
+    with open(prog, "w") as f:
+        f.write(text)
+    with tempfile.TemporaryDirectory() as td:
+        sf = os.path.join(td, "state.bin")
+        r = subprocess.run([NOISY_REF, prog, "--state", sf], capture_output=True, text=True, check=True,
+                           env=dict(os.environ, OMP_NUM_THREADS="1"))
+        st = np.fromfile(sf, dtype=np.complex128)
+    with open(os.path.join(HERE, "noisy_dp.nqres"), "w") as f:
+        f.write(r.stdout)
+    np.savez_compressed(os.path.join(HERE, "noisy_dp_state.npz"), state=st)
+    print(f"noisy_dp: {len(text.splitlines())} lines, norm {np.linalg.norm(st):.15f}")
+
+
 def main() -> None:
     if not os.path.exists(DRIVER):
         sys.exit(f"{DRIVER} missing: run `make -C oracle ref` first")
     make_io()
     make_noise()
+    make_noisy()
     make_qasm()
     for name, fn in CASES.items():
         c = fn()

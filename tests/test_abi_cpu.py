@@ -39,6 +39,8 @@ REFERENCE_METHODS = [
     "TurnOffSpecialize", "TurnOnSpecializeV2", "TurnOffSpecializeV2", "ResetRngPtr", "SetRngPtr",
     "SetSeedRngPtr", "GetProbability", "CollapseQubit", "Normalize", "AmplitudeWiseScalarMultiplication",
     "ExpectationValue", "ComputeNorm", "ComputeOverlap", "Print",
+    # noise (pybind11/intelqs_py.cpp:260-341)
+    "GetOverallSignOfChannels", "ApplyChannel", "SetNoiseTimescales", "ApplyNoiseGate",
 ]
 
 
@@ -46,7 +48,8 @@ def test_python_module_exposes_reference_names():
     import intelqs_py as m
     for n in ("QubitRegister", "RandomNumberGenerator", "MPIEnvironment", "EnvInit", "EnvFinalize",
               "InitializeVectorAsMaxCutCostFunction", "InitializeVectorAsWeightedMaxCutCostFunction",
-              "ImplementQaoaLayerBasedOnCostFunction", "GetExpectationValueFromCostFunction"):
+              "ImplementQaoaLayerBasedOnCostFunction", "GetExpectationValueFromCostFunction",
+              "CM4x4", "CM16x16", "NoisyQureg"):
         assert hasattr(m, n), n
     missing = [n for n in REFERENCE_METHODS if not hasattr(m.QubitRegister, n)]
     assert not missing, missing

@@ -264,6 +264,40 @@ int iqsh_hea_traffic(iqsh_hea_handle h, int flags, double *bytes);
 int iqsh_hea_autotune(iqsh_hea_handle h, int flags, int reps, iqsh_hea_trial *trials,
                       int max_trials, int *ntrials, double *best_ms);
 
+/* ---- pool of states ------------------------------------------------------------------
+ * A pool is an ordinary register of state_qubits + k positions: the top k positions index
+ * up to 2^k states of state_qubits qubits, state s holding amplitudes
+ * [s * 2^state_qubits, (s + 1) * 2^state_qubits).  Every gate entry point above, on
+ * positions < state_qubits, applies the same gate to every state in one launch.  The
+ * entry points below act state by state; states >= nstates (padding up to 2^k) are left
+ * untouched by them and reported by none.  Counterpart: the reference's pool of states,
+ * one state per MPI rank group (src/mpi_env.cpp:239-404), and
+ * Environment::IncoherentSumOverAllStatesOfPool (src/mpi_env.cpp:468-486), which sums one
+ * value per state -- the per-state values here. */
+enum iqsh_pool_reduce_kind {
+  IQSH_POOL_NORM2 = 0,   /* sum_i |psi_i|^2 */
+  IQSH_POOL_PROB1 = 1,   /* sum over i with bit `arg` set (GetProbability, unnormalized) */
+  IQSH_POOL_ZSTRING = 2  /* sum_i (-1)^popcount(i & arg) |psi_i|^2 (<Z...Z> on the mask) */
+};
+/* Every state s < nstates set to |index> (style 0) or to the uniform superposition
+ * (style 1, amplitude 2^(-state_qubits/2)); padding states zero.  Replaces
+ * Initialize("base"/"++++") run in every rank group (src/qureg_init.cpp:225-300). */
+int iqsh_pool_init(iqsh_handle h, unsigned state_qubits, int style, uint64_t index, uint64_t nstates);
+/* A different 2x2 matrix per state on position pos < state_qubits: mats holds nstates
+ * row-major complex 2x2 matrices (8 doubles each).  One launch for the whole pool; what the
+ * reference does with one Apply1QubitGate per rank group (noise gates and channel branches
+ * differ from state to state). */
+int iqsh_pool_apply_1q_each(iqsh_handle h, unsigned state_qubits, unsigned pos, const double *mats,
+                            uint64_t nstates);
+/* out[s] = the reduction `kind` of state s (FP64, fixed order), for s < nstates.  Replaces
+ * per-rank-group GetProbability / ComputeNorm / ExpectationValue of Z strings
+ * (src/qureg_measure.cpp:136-178, src/qureg_utils.cpp:237-262, src/qureg_expectval.cpp:104-375). */
+int iqsh_pool_reduce(iqsh_handle h, unsigned state_qubits, int kind, uint64_t arg, double *out, uint64_t nstates);
+/* out[2s], out[2s+1] = Re, Im of <ref|state s> = sum_i conj(ref_i) psi_{s,i}; ref is a
+ * register of state_qubits qubits and the pool's precision (ComputeOverlap per rank group,
+ * src/qureg_utils.cpp:268-299). */
+int iqsh_pool_overlap(iqsh_handle h, unsigned state_qubits, iqsh_handle ref, double *out, uint64_t nstates);
+
 #ifdef __cplusplus
 }
 #endif

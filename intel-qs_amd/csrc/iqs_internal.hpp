@@ -2,6 +2,21 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/iqs_hip.h"
+
+// A register: 2^n interleaved complex amplitudes in device memory.
+struct iqsh_state {
+  unsigned n = 0;
+  int dtype = IQSH_C128;
+  void* d = nullptr;        // amplitudes
+  bool owned = true;
+  void* spare = nullptr;    // second buffer for out-of-place permutation
+  uint64_t size() const { return uint64_t(1) << n; }
+  size_t esz() const { return dtype == IQSH_C128 ? 16 : 8; }
+};
 
 namespace iqsi {
 

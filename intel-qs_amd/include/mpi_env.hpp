@@ -34,7 +34,8 @@ class Environment
   static void RemapStateRank(int) {}
   static void UpdateStateComm(int num_states)
   {
-    if (num_states != 1) throw std::invalid_argument("a single-GPU environment holds exactly one state");
+    if (num_states != 1)
+      throw std::invalid_argument("one process holds one state here; a pool of states is iqs::StatePool (state_pool.hpp)");
   }
   template <class T>
   static T IncoherentSumOverAllStatesOfPool(T v) { return v; }

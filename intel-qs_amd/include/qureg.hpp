@@ -275,6 +275,23 @@ bool QubitRegister<Type>::do_print_extra_info = false;
 template <typename Type>
 using BaseType = typename QubitRegister<Type>::BaseType;
 
+/// The Pauli-twirl noise rotation of ApplyNoiseGate for an idle time `duration` under
+/// timescales T1, T2 (src/qureg_noisysimul.cpp:33-85): three standard normals drawn from
+/// the state stream of `rng` (X, Y, Z in that order).  Throws std::invalid_argument for
+/// non-positive noise probabilities.
+template <class Type>
+TM2x2<Type> NoiseGateMatrix(typename extract_value_type<Type>::value_type T1,
+                            typename extract_value_type<Type>::value_type T2,
+                            typename extract_value_type<Type>::value_type duration,
+                            RandomNumberGenerator<typename extract_value_type<Type>::value_type>& rng);
+
+/// One sampled branch of a 1-qubit channel, as ApplyChannel applies it
+/// (src/qureg_apply_channel.cpp:17-58): one uniform number from the state stream of `rng`;
+/// *negative tells whether the branch's eigenvalue is negative.
+template <class Type>
+TM2x2<Type> ChannelBranchMatrix(CM4x4<Type>& chi, RandomNumberGenerator<typename extract_value_type<Type>::value_type>& rng,
+                                bool* negative);
+
 }  // namespace iqs
 
 // Derived register that inserts noise gates automatically (the reference includes it here

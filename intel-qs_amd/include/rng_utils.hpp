@@ -25,12 +25,19 @@ class RandomNumberGenerator
   std::size_t GetSeed() const { return seed_; }
   void SetSeedStreamPtrs(std::size_t seed)
   {
+    SetSeedStreamPtrs(seed, mpi::Environment::GetStateId(), mpi::Environment::GetNumStates(),
+                      mpi::Environment::GetPoolRank());
+  }
+  /// The seeding a rank with these pool coordinates gets from SetSeedStreamPtrs(seed)
+  /// (src/rng_utils.cpp:84-99); used by iqs::StatePool to give each of its states the
+  /// streams of the corresponding pool state.
+  void SetSeedStreamPtrs(std::size_t seed, int state_id, int num_states, int pool_rank)
+  {
     seed_ = seed;
     n_local_ = n_state_ = n_pool_ = 0;
-    const int num_states = mpi::Environment::GetNumStates();
     pool_.seed(seed + 0);
-    state_.seed(seed + 1 + mpi::Environment::GetStateId());
-    local_.seed(seed + 1 + num_states + mpi::Environment::GetPoolRank());
+    state_.seed(seed + 1 + state_id);
+    local_.seed(seed + 1 + num_states + pool_rank);
   }
   void SkipAhead(std::size_t num_skip, std::string shared = "local")
   {

@@ -17,6 +17,7 @@
 
 #include "qaoa_features.hpp"
 #include "qureg.hpp"
+#include "state_pool.hpp"
 
 namespace py = pybind11;
 using iqs::QubitRegister;
@@ -104,7 +105,7 @@ void bind_rng(py::module_& m, const char* name)
   py::class_<R>(m, name)
       .def(py::init<>())
       .def("GetSeed", &R::GetSeed)
-      .def("SetSeedStreamPtrs", &R::SetSeedStreamPtrs)
+      .def("SetSeedStreamPtrs", py::overload_cast<std::size_t>(&R::SetSeedStreamPtrs))
       .def("SkipeAhead", &R::SkipAhead)
       .def("GetUniformRandomNumbers",
            [](R& r, std::size_t size, B a, B b, std::string shared) {
@@ -297,6 +298,39 @@ PYBIND11_MODULE(intelqs_py, m)
 
   bind_register<ComplexDP>(m, "QubitRegister");
   bind_register<ComplexSP>(m, "QubitRegisterSP");
+
+  // Extension: a pool of states in one GPU register (include/state_pool.hpp).
+  using SP = iqs::StatePool<ComplexDP>;
+  py::class_<SP>(m, "StatePool")
+      .def(py::init<unsigned, unsigned, std::string, std::size_t>(), py::arg("num_qubits"), py::arg("num_states"),
+           py::arg("style") = "base", py::arg("base_index") = 0)
+      .def("NumQubits", &SP::NumQubits)
+      .def("NumStates", &SP::NumStates)
+      .def("Register", &SP::Register, py::return_value_policy::reference_internal)
+      .def("Initialize", &SP::Initialize)
+      .def("SetSeedStreamPtrs", &SP::SetSeedStreamPtrs)
+      .def("SetNoiseTimescales", &SP::SetNoiseTimescales)
+      .def("ApplyNoiseGate", &SP::ApplyNoiseGate)
+      .def("ApplyChannel",
+           [](SP& a, unsigned qubit, const iqs::ChiMatrix<ComplexDP, 4, 32>& chi) {
+             CM4x4<ComplexDP> c(chi);
+             a.ApplyChannel(qubit, c);
+           })
+      .def("GetOverallSignOfChannels", &SP::GetOverallSignOfChannels)
+      .def("GetProbability", &SP::GetProbability)
+      .def("ComputeNorm", &SP::ComputeNorm)
+      .def("ExpectationValueZString", &SP::ExpectationValueZString)
+      .def("ComputeOverlap", &SP::ComputeOverlap)
+      .def("GetState",
+           [](SP& a, unsigned s) {
+             std::vector<ComplexDP> v = a.GetState(s);
+             return py::array_t<ComplexDP>(v.size(), v.data());
+           })
+      .def_static("IncoherentSumOverAllStatesOfPool", &SP::IncoherentSumOverAllStatesOfPool)
+      .def("__repr__", [](const SP& a) {
+        return "<StatePool of " + std::to_string(a.NumStates()) + " states of " + std::to_string(a.NumQubits()) +
+               " qubits in GPU memory>";
+      });
 
   // Extension: the C++ NoisyQureg (include/NoisyQureg.hpp) for Python callers.
   using NQ = iqs::NoisyQureg<ComplexDP>;

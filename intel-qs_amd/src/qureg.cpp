@@ -1233,10 +1233,12 @@ void QubitRegister<Type>::SetNoiseTimescales(BaseType T1, BaseType T2)
 }
 
 template <class Type>
-void QubitRegister<Type>::ApplyNoiseGate(const unsigned qubit, const BaseType duration)
+TM2x2<Type> NoiseGateMatrix(typename extract_value_type<Type>::value_type T_1_,
+                            typename extract_value_type<Type>::value_type T_2_,
+                            typename extract_value_type<Type>::value_type duration,
+                            RandomNumberGenerator<typename extract_value_type<Type>::value_type>& rng)
 {
-  if (!rng_ptr_) throw std::logic_error("ApplyNoiseGate: no random number generator (SetRngPtr)");
-  if (duration == 0) return;
+  using BaseType = typename extract_value_type<Type>::value_type;
   BaseType p_X, p_Y, p_Z;
   p_X = (1. - std::exp(-duration / T_1_)) / 4.;
   p_Y = (1. - std::exp(-duration / T_1_)) / 4.;
@@ -1247,11 +1249,11 @@ void QubitRegister<Type>::ApplyNoiseGate(const unsigned qubit, const BaseType du
   s_Y = std::sqrt(-std::log(1. - p_Y));
   s_Z = std::sqrt(-std::log(1. - p_Z));
   BaseType v_X, v_Y, v_Z;
-  rng_ptr_->GaussianRandomNumbers(&v_X, 1, "state");
+  rng.GaussianRandomNumbers(&v_X, 1, "state");
   v_X *= s_X / 2.;
-  rng_ptr_->GaussianRandomNumbers(&v_Y, 1, "state");
+  rng.GaussianRandomNumbers(&v_Y, 1, "state");
   v_Y *= s_Y / 2.;
-  rng_ptr_->GaussianRandomNumbers(&v_Z, 1, "state");
+  rng.GaussianRandomNumbers(&v_Z, 1, "state");
   v_Z *= s_Z / 2.;
   Type A, B, C;
   A = {std::cos(v_Z), -std::sin(v_Z)};
@@ -1262,7 +1264,15 @@ void QubitRegister<Type>::ApplyNoiseGate(const unsigned qubit, const BaseType du
   U(0, 1) = -std::conj(A) * std::conj(C);
   U(1, 0) = A * C;
   U(1, 1) = std::conj(A) * std::conj(B);
-  Apply1QubitGate(qubit, U);
+  return U;
+}
+
+template <class Type>
+void QubitRegister<Type>::ApplyNoiseGate(const unsigned qubit, const BaseType duration)
+{
+  if (!rng_ptr_) throw std::logic_error("ApplyNoiseGate: no random number generator (SetRngPtr)");
+  if (duration == 0) return;
+  Apply1QubitGate(qubit, NoiseGateMatrix<Type>(T_1_, T_2_, duration, *rng_ptr_));
 }
 
 // Branch index for a uniform draw r in [0, 1): the first k with r <= cumulative p_k
@@ -1283,11 +1293,11 @@ static unsigned channel_branch(const Chi& chi, double r, unsigned M)
 /// E_k = e0 id + e1 X + e2 Y + e3 Z as a 1-qubit gate, and flip the overall sign of
 /// channels for a negative eigenvalue.
 template <class Type>
-void QubitRegister<Type>::ApplyChannel(const unsigned qubit, CM4x4<Type>& chi)
+TM2x2<Type> ChannelBranchMatrix(CM4x4<Type>& chi, RandomNumberGenerator<typename extract_value_type<Type>::value_type>& rng,
+                                bool* negative)
 {
-  if (!rng_ptr_) throw std::logic_error("ApplyChannel: no random number generator (SetRngPtr)");
-  BaseType r;
-  rng_ptr_->UniformRandomNumbers(&r, 1, 0, 1, "state");
+  typename extract_value_type<Type>::value_type r;
+  rng.UniformRandomNumbers(&r, 1, 0, 1, "state");
   const unsigned k = channel_branch(chi, r, 4);
   const std::vector<Type> e = chi.GetEigenVector(k);
   const Type I(0., 1.);
@@ -1296,7 +1306,17 @@ void QubitRegister<Type>::ApplyChannel(const unsigned qubit, CM4x4<Type>& chi)
   m(0, 1) = e[1] - I * e[2];
   m(1, 0) = e[1] + I * e[2];
   m(1, 1) = e[0] - e[3];
-  if (std::real(chi.GetEigenValue(k)) < 0) overall_sign_of_channels *= -1;
+  *negative = std::real(chi.GetEigenValue(k)) < 0;
+  return m;
+}
+
+template <class Type>
+void QubitRegister<Type>::ApplyChannel(const unsigned qubit, CM4x4<Type>& chi)
+{
+  if (!rng_ptr_) throw std::logic_error("ApplyChannel: no random number generator (SetRngPtr)");
+  bool negative = false;
+  const TM2x2<Type> m = ChannelBranchMatrix(chi, *rng_ptr_, &negative);
+  if (negative) overall_sign_of_channels *= -1;
   Apply1QubitGate(qubit, m);
 }
 
@@ -1342,5 +1362,9 @@ void QubitRegister<Type>::ApplyChannel(const unsigned qubit1, const unsigned qub
 
 template class QubitRegister<ComplexSP>;
 template class QubitRegister<ComplexDP>;
+template TM2x2<ComplexSP> NoiseGateMatrix<ComplexSP>(float, float, float, RandomNumberGenerator<float>&);
+template TM2x2<ComplexDP> NoiseGateMatrix<ComplexDP>(double, double, double, RandomNumberGenerator<double>&);
+template TM2x2<ComplexSP> ChannelBranchMatrix<ComplexSP>(CM4x4<ComplexSP>&, RandomNumberGenerator<float>&, bool*);
+template TM2x2<ComplexDP> ChannelBranchMatrix<ComplexDP>(CM4x4<ComplexDP>&, RandomNumberGenerator<double>&, bool*);
 
 }  // namespace iqs

@@ -290,6 +290,55 @@ def make_qasm() -> None:
         print(f"{name}: {len(lines)} lines, stdout {len(r.stdout.splitlines())} lines, stderr {len(r.stderr.splitlines())}")
 
 
+POOL_STATES, POOL_SEED = 5, 77777
+
+
+def case_pool(seed: int) -> Circuit:
+    """One pool state's program in the reference's noisy-pool tutorial
+    (tutorials/get_started_with_noisy_IQS.cpp:160-215) at 6 qubits: noise after state
+    preparation, then X, Y and Z rotations qubit by qubit, each followed by a noise gate on
+    every qubit, a CNOT ladder in between, and the read-outs.  Pool state s of the
+    reference draws from the state stream seeded with seed + 1 + s, which a single-state run
+    gets from SetRngSeed(seed + s)."""
+    n = 6
+    ang = SplitMix64(4040)
+    angles = [[ang.uniform() * math.pi for _ in range(n)] for _ in range(3)]
+    c = Circuit(n, precision="dp")
+    c.add("SetRngSeed", seed)
+    c.add("SetNoiseTimescales", 40.0, 20.0)
+    for q in range(n):
+        c.add("ApplyNoiseGate", q, 1.5)
+    for axis, gate in enumerate(("ApplyRotationX", "ApplyRotationY", "ApplyRotationZ")):
+        for qubit in range(n):
+            c.add(gate, qubit, angles[axis][qubit])
+            for q in range(n):
+                c.add("ApplyNoiseGate", q, 1.5)
+        if axis == 0:
+            for q in range(n - 1):
+                c.add("ApplyCPauliX", q, q + 1)
+    c.add("GetProbability", 0)
+    c.add("GetProbability", 3)
+    return c
+
+
+def make_pool() -> None:
+    """pool_dp.poolcirc (the program of pool state 0), and per state s the reference's final
+    state and read-outs for seed POOL_SEED + s (pool_dp_states.npz, pool_dp.poolres)."""
+    states, res = [], []
+    for s in range(POOL_STATES):
+        text = case_pool(POOL_SEED + s).to_text()
+        if s == 0:
+            with open(os.path.join(HERE, "pool_dp.poolcirc"), "w") as f:
+                f.write(text)
+        st, r = run_reference(text)
+        states.append(st)
+        res.append(f"# state {s}\n" + r)
+    np.savez_compressed(os.path.join(HERE, "pool_dp_states.npz"), states=np.array(states), seed=POOL_SEED)
+    with open(os.path.join(HERE, "pool_dp.poolres"), "w") as f:
+        f.write("".join(res))
+    print(f"pool_dp: {POOL_STATES} states, {len(case_pool(0).ops)} ops each")
+
+
 NOISY_REF = os.path.join(ROOT, "oracle", "_ref", "noisy_driver")
 
 
@@ -338,6 +387,7 @@ def main() -> None:
     make_io()
     make_noise()
     make_noisy()
+    make_pool()
     make_qasm()
     for name, fn in CASES.items():
         c = fn()

@@ -115,6 +115,116 @@ def cpu_baseline(n: int, layer_ops, budget_s: float = 30.0):
                       f"(intel-qs src/, -O3 OpenMP), per-gate steady_clock"}
 
 
+def config1_against_reference(torch):
+    """The reference's CPU-runnable case: a 20-qubit ComplexDP brick circuit of depth 20
+    (seed 20240601) with <Z_i> read-outs, run through the drop-in on the GPU and through the
+    compiled reference (oracle/_ref/ref_driver, OpenMP) on this host.  Reports both wall
+    times and how far the results differ (the reference is the checker here)."""
+    import numpy as np
+    from iqs_amd import runner
+    from iqs_amd.circuits import brick_circuit
+
+    drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if not os.path.exists(drv):
+        return None
+    c = brick_circuit(20, 20, 20240601)
+    runner.run(c)  # warm-up (allocations, first launches)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reg, out = runner.run(c)
+    gpu_state = np.array(reg, copy=True)
+    gpu_s = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    with tempfile.TemporaryDirectory() as d:
+        cf, sf, rf = (os.path.join(d, x) for x in ("c.txt", "s.bin", "r.txt"))
+        c.write(cf)
+        t0 = time.perf_counter()
+        r = subprocess.run([drv, cf, "--state", sf, "--results", rf, "--time"], capture_output=True, text=True,
+                           timeout=600, env=dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread",
+                                                 OMP_PLACES="cores"))
+        cpu_wall = time.perf_counter() - t0
+        if r.returncode != 0:
+            return None
+        ref_state = np.fromfile(sf, dtype=np.complex128)
+        ref_out = [l for l in open(rf).read().splitlines() if l]
+    gate_s = sum(float(l.split()[3]) for l in r.stderr.splitlines() if l.startswith("T "))
+    zg = [float(x) for l in out for x in l.split()[2:]]
+    zr = [float(x) for l in ref_out for x in l.split()[2:]]
+    return {"workload": "20-qubit ComplexDP brick circuit, depth 20, seed 20240601, <Z_i> read-outs",
+            "gates": len(c.ops) - 1, "gpu_s": gpu_s, "cpu_gate_s": gate_s, "cpu_wall_s": cpu_wall, "cpu_threads": threads,
+            "max_abs_amplitude_diff": float(np.max(np.abs(gpu_state - ref_state))),
+            "max_abs_z_diff": float(max(abs(a - b) for a, b in zip(zg, zr))) if zg and len(zg) == len(zr) else None}
+
+
+def qaoa_param_shift_on_reference(torch, n: int = 16, p: int = 8):
+    """QAOA MaxCut gradient at 16 qubits, p = 8 (3-regular graph seed 20240604, angles seed
+    20240605): parameter-shift through the compiled reference on this host (one driver run
+    holding every shifted circuit: 2 evaluations per edge and per qubit and layer, the
+    reference has no gradient code) against the GPU adjoint on the same problem."""
+    import numpy as np
+    from iqs_amd.circuits import Circuit
+    from iqs_amd.qaoa import default_problem
+
+    drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if not os.path.exists(drv):
+        return None
+    prob, g, b = default_problem(n, p, "c128")
+    prob.adjoint(g, b)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cost, dg, db = prob.adjoint(g, b)
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    c = Circuit(n, precision="dp", init="plus", graph=list(prob.edges))
+    plan = []  # (kind, layer, coefficient) per pair of evaluations
+
+    def evaluation(shift):
+        c.add("Initialize", "++++", 0)
+        for l, (gl, bl) in enumerate(zip(g, b)):
+            c.add("QaoaPhase", float(gl))
+            if shift[0] == "edge" and shift[1] == l:
+                u, v, w = prob.edges[shift[2]]
+                ph = -shift[3] * w
+                c.add("ApplyDiag", u, v, 1.0, 0.0, math.cos(ph), math.sin(ph), math.cos(ph), math.sin(ph), 1.0, 0.0)
+            for q in range(n):
+                c.add("ApplyRotationX", q, 2.0 * float(bl))
+            if shift[0] == "qubit" and shift[1] == l:
+                c.add("ApplyRotationX", shift[2], 2.0 * shift[3])
+        c.add("QaoaExpectation")
+
+    for l in range(p):
+        for e, (_, _, w) in enumerate(prob.edges):
+            d = math.pi / (2.0 * w)
+            evaluation(("edge", l, e, d))
+            evaluation(("edge", l, e, -d))
+            plan.append(("gamma", l, 0.5 * w))
+        for q in range(n):
+            evaluation(("qubit", l, q, math.pi / 4))
+            evaluation(("qubit", l, q, -math.pi / 4))
+            plan.append(("beta", l, 1.0))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    with tempfile.TemporaryDirectory() as d:
+        cf, rf = os.path.join(d, "c.txt"), os.path.join(d, "r.txt")
+        c.write(cf)
+        t0 = time.perf_counter()
+        r = subprocess.run([drv, cf, "--results", rf], capture_output=True, text=True, timeout=900,
+                           env=dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread",
+                                    OMP_PLACES="cores"))
+        t_cpu = time.perf_counter() - t0
+        if r.returncode != 0:
+            return None
+        vals = [float(l.split()[2]) for l in open(rf) if l.startswith("R QaoaExpectation")]
+    grad = {("gamma", l): 0.0 for l in range(p)}
+    grad.update({("beta", l): 0.0 for l in range(p)})
+    for i, (kind, l, coef) in enumerate(plan):
+        grad[(kind, l)] += coef * (vals[2 * i] - vals[2 * i + 1])
+    diff = max(max(abs(grad[("gamma", l)] - dg[l]), abs(grad[("beta", l)] - db[l])) for l in range(p))
+    return {"workload": f"QAOA MaxCut {n} qubits, p = {p}, {len(prob.edges)} edges, complex128",
+            "evaluations": len(vals), "cpu_param_shift_s": t_cpu, "cpu_threads": threads,
+            "gpu_adjoint_s": t_adj, "cpu_param_shift_over_gpu_adjoint": t_cpu / t_adj,
+            "max_abs_diff_gpu_adjoint_vs_cpu_param_shift": diff}
+
+
 def qaoa_measure(n: int, p: int, full: bool, torch):
     """28-qubit QAOA MaxCut gradient: adjoint vs parameter-shift on the same GPU."""
     import numpy as np
@@ -421,6 +531,9 @@ def main() -> int:
     batch = None if args.no_batch else batch_measure(args.batch_reps, torch)
     log("HEA batch done")
     cpu = None if args.no_cpu else cpu_baseline(n, layers[0])
+    if cpu is not None:
+        cpu["config1_20q"] = config1_against_reference(torch)
+        cpu["qaoa_16q_param_shift"] = None if args.no_qaoa else qaoa_param_shift_on_reference(torch)
     log("CPU baseline done")
 
     out = {

@@ -139,6 +139,7 @@ struct Runner {
     else if (name == "ApplyNoiseGate") { double d; is >> a >> d; psi->ApplyNoiseGate(a, BT(d)); }
     else if (name == "Print") { std::string label; is >> label; std::fflush(res); psi->Print(label); std::fflush(stdout); }
     else if (name == "ExportAmplitudes") { std::string f; is >> f; psi->ExportAmplitudes(f); }
+    else if (name == "Initialize") { std::string st; std::size_t idx; is >> st >> idx; psi->Initialize(st, idx); }
     else if (name == "QaoaPhase") { is >> t; ensure_diag(); iqs::qaoa::ImplementQaoaLayerBasedOnCostFunction(*psi, *diag, BT(t)); }
     // ---- read-outs ----
     else if (name == "GetProbability") { is >> a; fprintf(res, "R GetProbability %u %.17g\n", a, (double)psi->GetProbability(a)); }

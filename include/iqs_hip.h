@@ -264,6 +264,20 @@ int iqsh_hea_traffic(iqsh_hea_handle h, int flags, double *bytes);
 int iqsh_hea_autotune(iqsh_hea_handle h, int flags, int reps, iqsh_hea_trial *trials,
                       int max_trials, int *ntrials, double *best_ms);
 
+/* ---- statistics ---------------------------------------------------------------------- */
+/* With p_i = |psi_i|^2, in FP64: out[0] = -sum p ln p, out[1] = -sum_{p > 0} ln p,
+ * out[k - 1] = sum p^k for k = 2..10.  Replaces the loops of QubitRegister::Entropy and
+ * ::GoogleStats (src/qureg_utils.cpp:306-450), which scale these sums on the host. */
+int iqsh_prob_moments(iqsh_handle h, double out[11]);
+/* hist[b] = sum of |psi_i|^2 over i whose cost value c_i = Re diag_i falls in bin b:
+ * mode 0 b = (int)c (GetHistogramFromCostFunction, src/qaoa_features.cpp:335-393),
+ * mode 1 b = floor(c + 1e-7) (...WithWeightsRounded, :402-462), mode 2
+ * b = floor(c / bin_width + 1e-7) (...WithWeightsBinned, :471-529).  A cost value outside
+ * [0, max_value] (mode 0) or [-1e-7, max_value + 1e-7] (modes 1, 2), or a bin outside
+ * [0, nbins), is an error (the reference asserts).  nbins <= 4096. */
+int iqsh_cost_histogram(iqsh_handle psi, iqsh_handle diag, int mode, double max_value, double bin_width,
+                        double *hist, int nbins);
+
 /* ---- pool of states ------------------------------------------------------------------
  * A pool is an ordinary register of state_qubits + k positions: the top k positions index
  * up to 2^k states of state_qubits qubits, state s holding amplitudes

@@ -34,5 +34,24 @@ template <typename Type>
 typename QubitRegister<Type>::BaseType GetExpectationValueSquaredFromCostFunction(const QubitRegister<Type>& psi,
                                                                                   const QubitRegister<Type>& diag);
 
+/// Probability of each integer cost value 0..max_value: bin (int)diag_i
+/// (src/qaoa_features.cpp:335-393).
+template <typename Type>
+std::vector<typename QubitRegister<Type>::BaseType> GetHistogramFromCostFunction(const QubitRegister<Type>& psi,
+                                                                                 const QubitRegister<Type>& diag,
+                                                                                 int max_value);
+
+/// Weighted costs rounded down: bin floor(diag_i + 1e-7), floor(max_value) + 1 bins
+/// (src/qaoa_features.cpp:402-462).
+template <typename Type>
+std::vector<typename QubitRegister<Type>::BaseType> GetHistogramFromCostFunctionWithWeightsRounded(
+    const QubitRegister<Type>& psi, const QubitRegister<Type>& diag, double max_value);
+
+/// Weighted costs in bins of bin_width: bin floor(diag_i / bin_width + 1e-7),
+/// ceil(max_value / bin_width) + 1 bins (src/qaoa_features.cpp:471-529).
+template <typename Type>
+std::vector<typename QubitRegister<Type>::BaseType> GetHistogramFromCostFunctionWithWeightsBinned(
+    const QubitRegister<Type>& psi, const QubitRegister<Type>& diag, double max_value, double bin_width);
+
 }  // namespace qaoa
 }  // namespace iqs

@@ -90,6 +90,20 @@ class QubitRegister
   // ---- permutation of the qubit order (src/qureg_permute.cpp) ----
   void PermuteQubits(std::vector<std::size_t> new_map, std::string style_of_map = "direct");
   void PermuteLocalQubits(std::vector<std::size_t> new_map, std::string style_of_map = "direct");
+  // With the whole state on one GPU every qubit is local: the reference's global-qubit
+  // permutations (src/qureg_permute.cpp:110-229) are PermuteQubits here.
+  void PermuteGlobalQubits(std::vector<std::size_t> new_map, std::string style_of_map = "direct")
+  {
+    PermuteQubits(new_map, style_of_map);
+  }
+  void PermuteByLocalGlobalExchangeOfQubitPairs(std::vector<std::size_t> new_map, std::string style_of_map = "direct")
+  {
+    PermuteQubits(new_map, style_of_map);
+  }
+  // One process holds the state: data rank = communicator rank = 0 (include/qureg.hpp:204-209).
+  int GetDataRank(int) const { return 0; }
+  int GetMyDataRank() const { return 0; }
+  int GetCommRank(int) const { return 0; }
   void EmulateSwap(unsigned qubit1, unsigned qubit2);
 
   // ---- one-qubit gates (src/qureg_apply1qubitgate.cpp) ----
@@ -194,6 +208,8 @@ class QubitRegister
   void AmplitudeWiseScalarMultiplication(Type factor);
   void AmplitudeWiseSum(QubitRegister<Type>& psi, Type factor = Type(1, 0));
   Type ComputeOverlap(QubitRegister<Type>& psi);
+  double Entropy();
+  std::vector<double> GoogleStats();
   void Print(std::string x, std::vector<std::size_t> qbits = {});
   void ExportAmplitudes(std::string ofname);
   void dumpbin(std::string fn);

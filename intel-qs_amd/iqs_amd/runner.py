@@ -98,6 +98,10 @@ def run(c: Circuit, reg=None) -> Tuple[object, List[str]]:
             out.append(f"R ExpectationValue {_fmt(reg.ExpectationValue(qs, os_, coeff))}")
         elif name == "ComputeNorm":
             out.append(f"R ComputeNorm {_fmt(reg.ComputeNorm())}")
+        elif name == "Entropy":
+            out.append(f"R Entropy {_fmt(reg.Entropy())}")
+        elif name == "GoogleStats":
+            out.append("R GoogleStats " + " ".join(_fmt(v) for v in reg.GoogleStats()))
         elif name == "GetGlobalAmplitude":
             v = reg.GetGlobalAmplitude(args[0])
             out.append(f"R GetGlobalAmplitude {args[0]} {_fmt(v.real)} {_fmt(v.imag)}")

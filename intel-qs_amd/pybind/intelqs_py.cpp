@@ -272,6 +272,9 @@ void bind_register(py::module_& m, const char* name)
              a.Print(description, {});
            },
            "Print the quantum state with an initial description.")
+      .def("Entropy", &Q::Entropy)
+      .def("GoogleStats", &Q::GoogleStats)
+      .def("PermuteGlobalQubits", &Q::PermuteGlobalQubits, py::arg("new_map"), py::arg("style_of_map") = "direct")
       .def("ExportAmplitudes", &Q::ExportAmplitudes)
       .def("dumpbin", &Q::dumpbin)
       .def("loadbin", &Q::loadbin)
@@ -376,6 +379,14 @@ PYBIND11_MODULE(intelqs_py, m)
         "Implement exp(-i gamma C)|psi>.");
   m.def("GetExpectationValueFromCostFunction", &iqs::qaoa::GetExpectationValueFromCostFunction<ComplexDP>,
         "Get expectation value from the cost function.");
+  m.def("GetHistogramFromCostFunction", &iqs::qaoa::GetHistogramFromCostFunction<ComplexDP>,
+        "Get histogram instead of just the expectation value.");
+  m.def("GetHistogramFromCostFunctionWithWeightsRounded",
+        &iqs::qaoa::GetHistogramFromCostFunctionWithWeightsRounded<ComplexDP>,
+        "Get histogram instead of just the expectation value for the weighted graphs, rounded down.");
+  m.def("GetHistogramFromCostFunctionWithWeightsBinned",
+        &iqs::qaoa::GetHistogramFromCostFunctionWithWeightsBinned<ComplexDP>,
+        "Get histogram instead of just the expectation value for the weighted graphs, with bins of given width.");
   m.def("GetExpectationValueSquaredFromCostFunction",
         &iqs::qaoa::GetExpectationValueSquaredFromCostFunction<ComplexDP>,
         "Get expectation value squared from the cost function.");

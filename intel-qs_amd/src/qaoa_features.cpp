@@ -1,6 +1,7 @@
 // QAOA MaxCut helpers of the drop-in, over the C ABI (see qaoa_features.hpp).
 #include "qaoa_features.hpp"
 
+#include <cmath>
 #include <stdexcept>
 #include <string>
 
@@ -99,6 +100,42 @@ typename QubitRegister<Type>::BaseType GetExpectationValueSquaredFromCostFunctio
   return (typename QubitRegister<Type>::BaseType)r;
 }
 
+template <typename Type>
+static std::vector<typename QubitRegister<Type>::BaseType> histogram(const QubitRegister<Type>& psi,
+                                                                     const QubitRegister<Type>& diag, int mode,
+                                                                     double max_value, double width, int nbins)
+{
+  same_layout(psi, diag);
+  if (nbins < 1) throw std::invalid_argument("cost histogram: max_value must be positive");
+  std::vector<double> h(nbins);
+  ck(iqsh_cost_histogram(const_cast<QubitRegister<Type>&>(psi).DeviceHandle(),
+                         const_cast<QubitRegister<Type>&>(diag).DeviceHandle(), mode, max_value, width, h.data(), nbins));
+  return std::vector<typename QubitRegister<Type>::BaseType>(h.begin(), h.end());
+}
+
+template <typename Type>
+std::vector<typename QubitRegister<Type>::BaseType> GetHistogramFromCostFunction(const QubitRegister<Type>& psi,
+                                                                                 const QubitRegister<Type>& diag,
+                                                                                 int max_value)
+{
+  return histogram(psi, diag, 0, (double)max_value, 1.0, max_value + 1);
+}
+
+template <typename Type>
+std::vector<typename QubitRegister<Type>::BaseType> GetHistogramFromCostFunctionWithWeightsRounded(
+    const QubitRegister<Type>& psi, const QubitRegister<Type>& diag, double max_value)
+{
+  return histogram(psi, diag, 1, max_value, 1.0, (int)std::floor(max_value) + 1);
+}
+
+template <typename Type>
+std::vector<typename QubitRegister<Type>::BaseType> GetHistogramFromCostFunctionWithWeightsBinned(
+    const QubitRegister<Type>& psi, const QubitRegister<Type>& diag, double max_value, double bin_width)
+{
+  if (!(bin_width > 0)) throw std::invalid_argument("cost histogram: bin_width must be positive");
+  return histogram(psi, diag, 2, max_value, bin_width, (int)std::ceil(max_value / bin_width) + 1);
+}
+
 template int InitializeVectorAsMaxCutCostFunction<ComplexDP>(QubitRegister<ComplexDP>&, std::vector<int>&);
 template int InitializeVectorAsMaxCutCostFunction<ComplexSP>(QubitRegister<ComplexSP>&, std::vector<int>&);
 template double InitializeVectorAsWeightedMaxCutCostFunction<ComplexDP>(QubitRegister<ComplexDP>&, std::vector<double>&);
@@ -109,6 +146,20 @@ template double GetExpectationValueFromCostFunction<ComplexDP>(const QubitRegist
 template float GetExpectationValueFromCostFunction<ComplexSP>(const QubitRegister<ComplexSP>&, const QubitRegister<ComplexSP>&);
 template double GetExpectationValueSquaredFromCostFunction<ComplexDP>(const QubitRegister<ComplexDP>&, const QubitRegister<ComplexDP>&);
 template float GetExpectationValueSquaredFromCostFunction<ComplexSP>(const QubitRegister<ComplexSP>&, const QubitRegister<ComplexSP>&);
+template std::vector<double> GetHistogramFromCostFunction<ComplexDP>(const QubitRegister<ComplexDP>&,
+                                                                    const QubitRegister<ComplexDP>&, int);
+template std::vector<float> GetHistogramFromCostFunction<ComplexSP>(const QubitRegister<ComplexSP>&,
+                                                                   const QubitRegister<ComplexSP>&, int);
+template std::vector<double> GetHistogramFromCostFunctionWithWeightsRounded<ComplexDP>(const QubitRegister<ComplexDP>&,
+                                                                                      const QubitRegister<ComplexDP>&, double);
+template std::vector<float> GetHistogramFromCostFunctionWithWeightsRounded<ComplexSP>(const QubitRegister<ComplexSP>&,
+                                                                                     const QubitRegister<ComplexSP>&, double);
+template std::vector<double> GetHistogramFromCostFunctionWithWeightsBinned<ComplexDP>(const QubitRegister<ComplexDP>&,
+                                                                                     const QubitRegister<ComplexDP>&,
+                                                                                     double, double);
+template std::vector<float> GetHistogramFromCostFunctionWithWeightsBinned<ComplexSP>(const QubitRegister<ComplexSP>&,
+                                                                                    const QubitRegister<ComplexSP>&,
+                                                                                    double, double);
 
 }  // namespace qaoa
 }  // namespace iqs

@@ -1360,6 +1360,34 @@ void QubitRegister<Type>::ApplyChannel(const unsigned qubit1, const unsigned qub
   Apply2QubitGate(qubit1, qubit2, m);
 }
 
+/// -sum p log2 p over the basis probabilities (src/qureg_utils.cpp:306-345).
+template <class Type>
+double QubitRegister<Type>::Entropy()
+{
+  double m[11];
+  ck(iqsh_prob_moments(dev_(), m));
+  return m[0] / (double)std::log(2.0);
+}
+
+/// [entropy (bits), average self-information (bits) / 2^n, then for k = 2..10 the moments
+/// sum p^k scaled by 2^(n (k - 1)) / k!] (src/qureg_utils.cpp:352-450).
+template <class Type>
+std::vector<double> QubitRegister<Type>::GoogleStats()
+{
+  double m[11];
+  ck(iqsh_prob_moments(dev_(), m));
+  const double two2n = double(GlobalSize());
+  std::vector<double> stats;
+  stats.push_back(m[0] / (double)std::log(2.0));
+  stats.push_back(m[1] / (double)std::log(2.0) / two2n);
+  double factorial = 1.0;
+  for (int k = 2; k <= 10; ++k) {
+    factorial *= double(k);
+    stats.push_back(m[k] * (std::pow(two2n, double(k - 1)) / factorial));
+  }
+  return stats;
+}
+
 template class QubitRegister<ComplexSP>;
 template class QubitRegister<ComplexDP>;
 template TM2x2<ComplexSP> NoiseGateMatrix<ComplexSP>(float, float, float, RandomNumberGenerator<float>&);

@@ -151,6 +151,12 @@ struct Runner {
       fprintf(res, "R ExpectationValue %.17g\n", v);
     }
     else if (name == "ComputeNorm") { fprintf(res, "R ComputeNorm %.17g\n", (double)psi->ComputeNorm()); }
+    else if (name == "Entropy") { fprintf(res, "R Entropy %.17g\n", psi->Entropy()); }
+    else if (name == "GoogleStats") {
+      fprintf(res, "R GoogleStats");
+      for (double v : psi->GoogleStats()) fprintf(res, " %.17g", v);
+      fprintf(res, "\n");
+    }
     else if (name == "GetGlobalAmplitude") {
       std::size_t k; is >> k;
       Type v = psi->GetGlobalAmplitude(k);

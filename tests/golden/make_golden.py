@@ -290,6 +290,29 @@ def make_qasm() -> None:
         print(f"{name}: {len(lines)} lines, stdout {len(r.stdout.splitlines())} lines, stderr {len(r.stderr.splitlines())}")
 
 
+def make_stats() -> None:
+    """stats_<prec>.statcirc and the reference's Entropy / GoogleStats read-outs
+    (.statres): a 10-qubit brick state and a sparse one (three basis states, so that zero
+    probabilities are skipped as the reference skips them)."""
+    for prec in ("dp", "sp"):
+        c = brick_circuit(10, 6, 1234, precision=prec, readout=False)
+        c.add("Entropy")
+        c.add("GoogleStats")
+        sparse = Circuit(10, precision=prec)
+        sparse.add("ApplyHadamard", 3)
+        sparse.add("ApplyCHadamard", 3, 7)
+        sparse.add("Entropy")
+        sparse.add("GoogleStats")
+        for tag, circ in (("", c), ("_sparse", sparse)):
+            text = circ.to_text()
+            _, res = run_reference(text)
+            with open(os.path.join(HERE, f"stats{tag}_{prec}.statcirc"), "w") as f:
+                f.write(text)
+            with open(os.path.join(HERE, f"stats{tag}_{prec}.statres"), "w") as f:
+                f.write(res)
+            print(f"stats{tag}_{prec}: {res.splitlines()[0][:60]}")
+
+
 POOL_STATES, POOL_SEED = 5, 77777
 
 
@@ -388,6 +411,7 @@ def main() -> None:
     make_noise()
     make_noisy()
     make_pool()
+    make_stats()
     make_qasm()
     for name, fn in CASES.items():
         c = fn()

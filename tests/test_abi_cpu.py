@@ -49,7 +49,8 @@ def test_python_module_exposes_reference_names():
     for n in ("QubitRegister", "RandomNumberGenerator", "MPIEnvironment", "EnvInit", "EnvFinalize",
               "InitializeVectorAsMaxCutCostFunction", "InitializeVectorAsWeightedMaxCutCostFunction",
               "ImplementQaoaLayerBasedOnCostFunction", "GetExpectationValueFromCostFunction",
-              "CM4x4", "CM16x16", "NoisyQureg"):
+              "CM4x4", "CM16x16", "NoisyQureg", "GetHistogramFromCostFunction",
+              "GetHistogramFromCostFunctionWithWeightsRounded", "GetHistogramFromCostFunctionWithWeightsBinned"):
         assert hasattr(m, n), n
     missing = [n for n in REFERENCE_METHODS if not hasattr(m.QubitRegister, n)]
     assert not missing, missing

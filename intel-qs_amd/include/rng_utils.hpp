@@ -6,6 +6,7 @@
 #pragma once
 
 #include <cassert>
+#include <cmath>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -55,12 +56,16 @@ class RandomNumberGenerator
     std::mt19937* g = select(2 * size, shared);
     for (std::size_t i = 0; i < size; ++i) out[i] = n_(*g);
   }
-  template <typename IntType>
-  void RandomIntegersInRange(IntType* out, std::size_t size, IntType a, IntType b, std::string shared = "local")
+  /// Integers in [a, b): floor(a + u (b - a)) with u from the uniform stream, as the
+  /// reference computes them (src/rng_utils.cpp:170-188).
+  void RandomIntegersInRange(int* out, std::size_t size = 1, int a = 0, int b = 2, std::string shared = "local")
   {
     std::mt19937* g = select(size, shared);
-    std::uniform_int_distribution<IntType> d(a, b - 1);
-    for (std::size_t i = 0; i < size; ++i) out[i] = d(*g);
+    for (std::size_t i = 0; i < size; ++i) {
+      Type r = u_(*g);
+      r = (Type)a + r * Type(b - a);
+      out[i] = (int)std::floor(r);
+    }
   }
 
  private:

@@ -113,6 +113,25 @@ void bind_rng(py::module_& m, const char* name)
              r.UniformRandomNumbers(v.data(), size, a, b, shared);
              return v;
            })
+      // The reference binds these with raw pointers; here they fill a writable numpy array.
+      .def("UniformRandomNumbers",
+           [](R& r, py::array_t<B, py::array::c_style> out, std::size_t size, B a, B b, std::string shared) {
+             if ((std::size_t)out.size() < size) throw std::runtime_error("output array too small");
+             r.UniformRandomNumbers(out.mutable_data(), size, a, b, shared);
+           },
+           py::arg("out"), py::arg("size") = 1, py::arg("a") = B(0), py::arg("b") = B(1), py::arg("shared") = "local")
+      .def("GaussianRandomNumbers",
+           [](R& r, py::array_t<B, py::array::c_style> out, std::size_t size, std::string shared) {
+             if ((std::size_t)out.size() < size) throw std::runtime_error("output array too small");
+             r.GaussianRandomNumbers(out.mutable_data(), size, shared);
+           },
+           py::arg("out"), py::arg("size") = 1, py::arg("shared") = "local")
+      .def("RandomIntegersInRange",
+           [](R& r, py::array_t<int, py::array::c_style> out, std::size_t size, int a, int b, std::string shared) {
+             if ((std::size_t)out.size() < size) throw std::runtime_error("output array too small");
+             r.RandomIntegersInRange(out.mutable_data(), size, a, b, shared);
+           },
+           py::arg("out"), py::arg("size") = 1, py::arg("a") = 0, py::arg("b") = 2, py::arg("shared") = "local")
       .def("__repr__", [name](const R&) { return std::string("<") + name + ">"; });
 }
 

@@ -68,3 +68,34 @@ def test_rng_matches_std_mt19937_streams():
     r2 = m.RandomNumberGenerator()
     r2.SetSeedStreamPtrs(777)
     assert r2.GetUniformRandomNumbers(4, 0.0, 1.0, "pool") == v
+
+
+def _mt_uniform(seed, k):
+    """std::mt19937(seed) through libstdc++'s generate_canonical<double, 53>: two 32-bit
+    outputs per number, (g0 + g1 2^32) / 2^64 -- restated with numpy's MT19937 under the
+    same (legacy) initialization."""
+    import numpy as np
+    bg = np.random.MT19937(0)
+    bg._legacy_seeding(seed)
+    raw = bg.random_raw(2 * k)
+    return [(int(raw[2 * i]) + int(raw[2 * i + 1]) * 2.0 ** 32) / 2.0 ** 64 for i in range(k)]
+
+
+def test_rng_streams_and_integers_match_restatement():
+    """Streams seeded as src/rng_utils.cpp:84-99 (pool: seed, state: seed + 1 + state id,
+    local: seed + 1 + num_states + pool rank) and integers floor(a + u (b - a))
+    (src/rng_utils.cpp:170-188), bit for bit."""
+    import numpy as np
+    import intelqs_py as m
+    r = m.RandomNumberGenerator()
+    r.SetSeedStreamPtrs(777)
+    assert r.GetUniformRandomNumbers(5, 0.0, 1.0, "pool") == _mt_uniform(777, 5)
+    assert r.GetUniformRandomNumbers(3, -2.0, 6.0, "local") == [-2.0 + 8.0 * u for u in _mt_uniform(779, 3)]
+    out = np.zeros(8, dtype=np.int32)
+    r.RandomIntegersInRange(out, 8, 3, 13, "state")
+    assert list(out) == [int(np.floor(3 + u * 10)) for u in _mt_uniform(778, 8)]
+    buf = np.zeros(4)
+    r2 = m.RandomNumberGenerator()
+    r2.SetSeedStreamPtrs(777)
+    r2.UniformRandomNumbers(buf, 4, 0.0, 1.0, "pool")
+    assert list(buf) == _mt_uniform(777, 4)

@@ -47,6 +47,22 @@ __device__ __forceinline__ void stg(const V& v, V* p)
 #endif
 }
 
+// Access with a chosen cache policy: NT = true as ldg/stg; NT = false ordinary cached
+// accesses, for patterns where several instructions share each cache line (a line fetched
+// or written partially by one instruction is completed by the next through L2).
+template <bool NT, typename V>
+__device__ __forceinline__ V ldx(const V* p)
+{
+  if (NT) return ldg(p);
+  return *p;
+}
+template <bool NT, typename V>
+__device__ __forceinline__ void stx(const V& v, V* p)
+{
+  if (NT) stg(v, p);
+  else *p = v;
+}
+
 template <typename T> struct Cx { T r, i; };
 
 template <int K> struct Ins { unsigned p[K > 0 ? K : 1]; };
@@ -522,7 +538,12 @@ __device__ __forceinline__ MfmaAcc<double>::t mfma16x16x4(double a, double b, Mf
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, d, 0, 0, 0);
 }
 
-template <typename T, int TPW>
+// NT: non-temporal accesses.  When the lowest block position lies inside a cache line, the
+// rows of one group share lines and each load/store instruction covers part of every line
+// it touches; with cached accesses L2 completes the lines (30-qubit complex64, positions
+// {0,1,9,17,25}: 1.15 -> 4.1 TB/s; {1,9,15,22,29}: 2.9 -> 4.2 TB/s), while spread positions
+// run the same either way (5.3 TB/s).
+template <typename T, int TPW, bool NT>
 __global__ __launch_bounds__(kBlock) void k_block5_mfma(typename Vec<T>::t* __restrict__ st, uint64_t ntiles, Ins<5> ins,
                                                         const typename Vec<T>::t* __restrict__ U)
 {
@@ -566,14 +587,14 @@ __global__ __launch_bounds__(kBlock) void k_block5_mfma(typename Vec<T>::t* __re
   uint64_t base = item_base(t0 * 16 + gl, ins);
   V x[8];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) x[j] = ldg(&st[base | offin[j]]);
+  for (int j = 0; j < 8; ++j) x[j] = ldx<NT>(&st[base | offin[j]]);
   for (uint64_t t = t0; t < t1; ++t) {
     const bool more = t + 1 < t1;
     const uint64_t nbase = more ? item_base((t + 1) * 16 + gl, ins) : base;
     V xn[8];
     if (more) {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) xn[j] = ldg(&st[nbase | offin[j]]);
+      for (int j = 0; j < 8; ++j) xn[j] = ldx<NT>(&st[nbase | offin[j]]);
     }
     Acc d[4];
 #pragma unroll
@@ -591,8 +612,8 @@ __global__ __launch_bounds__(kBlock) void k_block5_mfma(typename Vec<T>::t* __re
       y0.y = d[2][i];
       y1.x = d[1][i];
       y1.y = d[3][i];
-      stg(y0, &st[base | offout[i]]);
-      stg(y1, &st[base | offout[4 + i]]);
+      stx<NT>(y0, &st[base | offout[i]]);
+      stx<NT>(y1, &st[base | offout[4 + i]]);
     }
     if (more) {
 #pragma unroll

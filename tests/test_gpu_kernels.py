@@ -363,6 +363,24 @@ def test_tiled_swap_diag_2q(dt, big_states, p1, p2):
     assert np.abs(sv.download() - h).max() < 4 * TOL[dt]
 
 
+@pytest.mark.parametrize("p,q", [(0, 9), (1, 0), (2, 16), (3, 1), (5, 2), (12, 3)])
+def test_tiled_unit_diagonals_at_low_positions(dt, big_states, p, q):
+    """Diagonals with entries exactly 1 (Z/S/T, CPhase, CZ): a position inside a cache line
+    goes through the full elementwise pass, elsewhere only the changed amplitudes move."""
+    h = big_states[dt].copy()
+    sv = on_gpu(h, dt)
+    f = np.exp(0.7j)
+    for m in (np.diag([1, f]), np.diag([f, 1]), np.diag([1, -1])):
+        sv.apply_1q(p, m)
+        oracle.apply_1q(h, p, m)
+    sv.apply_c1q(q, p, np.diag([1, f]))
+    oracle.apply_c1q(h, q, p, np.diag([1, f]))
+    for d in (np.array([1, 1, 1, f]), np.array([1, f, 1, 1]), np.array([f, 1, 1, 1])):
+        sv.apply_diag2(p, q, d)
+        oracle.apply_diag(h, p, q, d)
+    assert np.abs(sv.download() - h).max() < 4 * TOL[dt]
+
+
 def test_tiled_toffoli_and_scale(dt, big_states):
     h = big_states[dt].copy()
     sv = on_gpu(h, dt)

@@ -168,7 +168,12 @@ int launch_pair_m(iqsh_handle h, const Ins<K>& ins, uint64_t mask0, uint64_t mas
   // Complex128 with every touched position <= 10: one contiguous range per block (30q sweep:
   // dense 1q targets 6-10 5.9 -> 6.2 TB/s, controlled pairs below 11 5.8-6.2 -> 6.1-6.4);
   // split blocks everywhere else.
-  if (items >= kTiledMin && sizeof(T) == 8 && ins.p[K - 1] <= 10)
+  if (items >= kTiledMin && K >= 2 && ins.p[0] < line_bits<T>())
+    // Lowest position inside a cache line (a control above it): cached accesses let L2
+    // complete the lines that the two halves of each pair share.
+    k_pair<T, K, MOVE, true, kTB, kTP, kSub, kSubLog, false><<<grid2(items / (kTB * kTP)), kTB, 0, g_stream>>>(
+        (V*)h->d, items, ins, mask0, mask1, mat2<T>(m));
+  else if (items >= kTiledMin && sizeof(T) == 8 && ins.p[K - 1] <= 10)
     k_pair<T, K, MOVE, true, kTB, kTP, 1><<<grid2(items / (kTB * kTP)), kTB, 0, g_stream>>>((V*)h->d, items, ins, mask0,
                                                                                                     mask1, mat2<T>(m));
   else if (items >= kTiledMin)

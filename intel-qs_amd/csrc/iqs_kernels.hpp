@@ -147,7 +147,9 @@ __device__ __forceinline__ V cmul(const Cx<T>& a, const V& x)
 // MOVE=true exchanges the two amplitudes (X-type matrices: a pure, bit-exact move).
 // TILED=true: XCD-contiguous, split-block tiling (nitems a power of two >= kTiledMin, grid
 // exactly nitems / (B*P)); TILED=false: plain contiguous blocks with bounds checks.
-template <typename T, int K, bool MOVE, bool TILED, int B, int P, int SUB = kSub, unsigned SL = kSubLog>
+// NT = false: cached accesses, for a lowest position inside a cache line (the two halves of
+// every line are read and written by different instructions).
+template <typename T, int K, bool MOVE, bool TILED, int B, int P, int SUB = kSub, unsigned SL = kSubLog, bool NT = true>
 __global__ __launch_bounds__(B) void k_pair(typename Vec<T>::t* __restrict__ st, uint64_t nitems, Ins<K> ins,
                                             uint64_t mask0, uint64_t mask1, Mat2<T> m)
 {
@@ -159,17 +161,17 @@ __global__ __launch_bounds__(B) void k_pair(typename Vec<T>::t* __restrict__ st,
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       base[j] = item_base(tiled_item<B, P, SUB, SL>(bb, j * B + threadIdx.x), ins);
-      a[j] = ldg(&st[base[j] | mask0]);
-      b[j] = ldg(&st[base[j] | mask1]);
+      a[j] = ldx<NT>(&st[base[j] | mask0]);
+      b[j] = ldx<NT>(&st[base[j] | mask1]);
     }
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       if (MOVE) {
-        stg(b[j], &st[base[j] | mask0]);
-        stg(a[j], &st[base[j] | mask1]);
+        stx<NT>(b[j], &st[base[j] | mask0]);
+        stx<NT>(a[j], &st[base[j] | mask1]);
       } else {
-        stg(cmad2(m.m00, a[j], m.m01, b[j]), &st[base[j] | mask0]);
-        stg(cmad2(m.m10, a[j], m.m11, b[j]), &st[base[j] | mask1]);
+        stx<NT>(cmad2(m.m00, a[j], m.m01, b[j]), &st[base[j] | mask0]);
+        stx<NT>(cmad2(m.m10, a[j], m.m11, b[j]), &st[base[j] | mask1]);
       }
     }
   } else {

@@ -182,6 +182,17 @@ int iqsh_qaoa_phase_adjoint(iqsh_handle psi, iqsh_handle lam, unsigned ne, const
 int iqsh_adjoint_1q(iqsh_handle psi, iqsh_handle lam, unsigned pos, const double Uinv[8],
                     const double *G, double *grad);
 
+/* ---- layers of one gate on every qubit -------------------------------------------------
+ * psi := m (x) m (x) ... (x) m psi (the same 2x2 matrix on every position) in
+ * 1 + ceil((n - 6) / 4) passes instead of n.  Replaces the per-qubit loop that applies a
+ * QAOA mixer, RX(2 beta) on every qubit, after ImplementQaoaLayerBasedOnCostFunction
+ * (src/qaoa_features.cpp:73-110 users; src/qureg_apply1qubitgate.cpp:277-285).  n >= 10. */
+int iqsh_apply_1q_all(iqsh_handle h, const double m[8]);
+/* Backward step through such a layer for both states: *grad := sum_q 2 Im <lam| G_q |psi>
+ * (G on position q, each term as iqsh_adjoint_1q would take it), then psi := Uinv^(x)n psi,
+ * lam := Uinv^(x)n lam.  n >= 10. */
+int iqsh_adjoint_1q_all(iqsh_handle psi, iqsh_handle lam, const double Uinv[8], const double *G, double *grad);
+
 /* ---- batches of hardware-efficient-ansatz circuits ------------------------------------
  * No counterpart in the reference; the nearest is its pool of independent states
  * (src/mpi_env.cpp:239-404, one state per MPI rank group).  A batch holds `count`

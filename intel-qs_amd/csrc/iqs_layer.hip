@@ -1,0 +1,289 @@
+// Layers of one 2x2 gate on every qubit (U^(x)n), forward and adjoint: the QAOA mixer
+// exp(-i beta sum_q X_q) = prod_q RX_q(2 beta) and its backward step.  Entry points in
+// include/iqs_hip.h ("layers").
+//
+// Instead of n passes over the state (one per qubit), a layer takes
+//   * one lane-shuffle pass for positions 0..5: each lane holds one amplitude, partners at
+//     lane ^ 2^q come through __shfl_xor, every load/store is a contiguous 1 KiB run;
+//   * passes of up to four positions each above that: a thread holds the 2^m amplitudes of
+//     one group in registers (consecutive threads take consecutive groups, so every
+//     instruction is a contiguous run) and applies the m rotations there.
+// At 28 qubits: 7 passes instead of 28.
+//
+// Adjoint: the gates of a layer act on different qubits, so each generator G_q commutes
+// with every gate of the layer except its own, and inserting G_q after U_q equals inserting
+// it after the whole layer.  Each pass therefore takes its qubits' terms
+// 2 Im <lam| G_q |psi> on the states it holds, then applies Uinv to them.  Sums in FP64:
+// per-workgroup partials, summed in order on the host (bitwise reproducible).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "../../include/iqs_hip.h"
+#include "iqs_internal.hpp"
+#include "iqs_kernels.hpp"
+
+using namespace iqsk;
+
+namespace {
+
+constexpr int kLB = 256;          // threads per workgroup
+constexpr int kLowQ = 6;          // positions handled by the lane-shuffle pass
+constexpr int kLowP = 4;          // amplitudes per lane in the lane-shuffle pass
+constexpr int kRegQ = 4;          // positions per register pass
+constexpr unsigned kMaxGrid = 2048;
+
+struct LayerWork {
+  double* part = nullptr;
+  std::vector<double> host;
+};
+LayerWork g_lw;
+std::mutex g_lm;
+
+int part_buf()
+{
+  if (!g_lw.part && hipMalloc((void**)&g_lw.part, sizeof(double) * kMaxGrid) != hipSuccess)
+    return iqsi::fail(IQSH_ERR_ALLOC, "layer reduction buffer");
+  return IQSH_OK;
+}
+
+template <typename T, typename V>
+__device__ __forceinline__ V mat_row(const Cx<T>& a, const V& x0, const Cx<T>& b, const V& x1)
+{
+  return cmad2(a, x0, b, x1);
+}
+
+// Im(conj(y) * (a x0 + b x1)) with the product in FP64.
+template <typename V>
+__device__ __forceinline__ double im_term(const V& y, const Cx<double>& a, const V& x0, const Cx<double>& b, const V& x1)
+{
+  const double gr = (a.r * (double)x0.x - a.i * (double)x0.y) + (b.r * (double)x1.x - b.i * (double)x1.y);
+  const double gi = (a.r * (double)x0.y + a.i * (double)x0.x) + (b.r * (double)x1.y + b.i * (double)x1.x);
+  return (double)y.x * gi - (double)y.y * gr;
+}
+
+__device__ __forceinline__ void wg_sum_store(double v, double* out)
+{
+  __shared__ double sh[kLB / 64];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int w = 0; w < kLB / 64; ++w) s += sh[w];
+    out[blockIdx.x] = s;
+  }
+}
+
+template <typename V>
+__device__ __forceinline__ V shfl_v(const V& a, int mask)
+{
+  V r;
+  r.x = __shfl_xor(a.x, mask);
+  r.y = __shfl_xor(a.y, mask);
+  return r;
+}
+
+// Lane-shuffle pass over positions 0..L-1 (L <= 6).  N a multiple of kLB * kLowP.
+template <typename T, bool ADJ>
+__global__ __launch_bounds__(kLB) void k_layer_low(typename Vec<T>::t* __restrict__ psi, typename Vec<T>::t* __restrict__ lam,
+                                                   uint64_t N, int L, Mat2<T> u, Mat2<double> g, double* __restrict__ part)
+{
+  using V = typename Vec<T>::t;
+  const int lane = threadIdx.x & 63;
+  double acc = 0;
+  for (uint64_t r0 = (uint64_t)blockIdx.x * (kLB * kLowP) + threadIdx.x; r0 < N; r0 += (uint64_t)gridDim.x * (kLB * kLowP)) {
+    V x[kLowP], y[kLowP];
+#pragma unroll
+    for (int j = 0; j < kLowP; ++j) {
+      x[j] = ldg(&psi[r0 + (uint64_t)j * kLB]);
+      if (ADJ) y[j] = ldg(&lam[r0 + (uint64_t)j * kLB]);
+    }
+    for (int q = 0; q < L; ++q) {
+      const int msk = 1 << q;
+      const bool hi = (lane >> q) & 1;
+#pragma unroll
+      for (int j = 0; j < kLowP; ++j) {
+        const V xp = shfl_v(x[j], msk);
+        const V x0 = hi ? xp : x[j], x1 = hi ? x[j] : xp;
+        if (ADJ) {
+          acc += hi ? im_term(y[j], g.m10, x0, g.m11, x1) : im_term(y[j], g.m00, x0, g.m01, x1);
+          const V yp = shfl_v(y[j], msk);
+          const V y0 = hi ? yp : y[j], y1 = hi ? y[j] : yp;
+          y[j] = hi ? mat_row(u.m10, y0, u.m11, y1) : mat_row(u.m00, y0, u.m01, y1);
+        }
+        x[j] = hi ? mat_row(u.m10, x0, u.m11, x1) : mat_row(u.m00, x0, u.m01, x1);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kLowP; ++j) {
+      stg(x[j], &psi[r0 + (uint64_t)j * kLB]);
+      if (ADJ) stg(y[j], &lam[r0 + (uint64_t)j * kLB]);
+    }
+  }
+  if (ADJ) wg_sum_store(acc, part);
+}
+
+// Register pass over M positions ins (all >= 6): each thread holds the 2^M amplitudes of a
+// group.  The generator terms of a qubit are taken before any of this pass's gates.
+template <typename T, bool ADJ, int M>
+__global__ __launch_bounds__(kLB) void k_layer_reg(typename Vec<T>::t* __restrict__ psi, typename Vec<T>::t* __restrict__ lam,
+                                                   uint64_t items, Ins<M> ins, Mat2<T> u, Mat2<double> g,
+                                                   double* __restrict__ part)
+{
+  using V = typename Vec<T>::t;
+  constexpr int D = 1 << M;
+  uint64_t off[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int b = 0; b < M; ++b)
+      if ((j >> b) & 1) o |= uint64_t(1) << ins.p[b];
+    off[j] = o;
+  }
+  double acc = 0;
+  for (uint64_t k = (uint64_t)blockIdx.x * kLB + threadIdx.x; k < items; k += (uint64_t)gridDim.x * kLB) {
+    const uint64_t base = item_base(k, ins);
+    V x[D], y[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      x[j] = ldg(&psi[base | off[j]]);
+      if (ADJ) y[j] = ldg(&lam[base | off[j]]);
+    }
+    if (ADJ) {
+#pragma unroll
+      for (int b = 0; b < M; ++b)
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          if ((j >> b) & 1) continue;
+          const int j1 = j | (1 << b);
+          acc += im_term(y[j], g.m00, x[j], g.m01, x[j1]) + im_term(y[j1], g.m10, x[j], g.m11, x[j1]);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < M; ++b)
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        if ((j >> b) & 1) continue;
+        const int j1 = j | (1 << b);
+        const V a = x[j], c = x[j1];
+        x[j] = mat_row(u.m00, a, u.m01, c);
+        x[j1] = mat_row(u.m10, a, u.m11, c);
+        if (ADJ) {
+          const V ya = y[j], yc = y[j1];
+          y[j] = mat_row(u.m00, ya, u.m01, yc);
+          y[j1] = mat_row(u.m10, ya, u.m11, yc);
+        }
+      }
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      stg(x[j], &psi[base | off[j]]);
+      if (ADJ) stg(y[j], &lam[base | off[j]]);
+    }
+  }
+  if (ADJ) wg_sum_store(acc, part);
+}
+
+unsigned grid_for_work(uint64_t work_items)
+{
+  return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((work_items + kLB - 1) / kLB, kMaxGrid));
+}
+
+// Sum the partials of one adjoint pass (in order) into *acc.
+int finish_pass(unsigned grid, double* acc)
+{
+  IQSI_TRY(hipGetLastError());
+  g_lw.host.resize(grid);
+  IQSI_TRY(hipMemcpyAsync(g_lw.host.data(), g_lw.part, sizeof(double) * grid, hipMemcpyDeviceToHost, iqsi::stream()));
+  IQSI_TRY(hipStreamSynchronize(iqsi::stream()));
+  double s = 0;
+  for (unsigned b = 0; b < grid; ++b) s += g_lw.host[b];
+  *acc += s;
+  return IQSH_OK;
+}
+
+template <typename T, bool ADJ, int M>
+int reg_pass(iqsh_handle h, iqsh_handle lam, unsigned p0, const Mat2<T>& u, const Mat2<double>& g, double* acc)
+{
+  using V = typename Vec<T>::t;
+  Ins<M> ins;
+  for (int b = 0; b < M; ++b) ins.p[b] = p0 + b;
+  const uint64_t items = h->size() >> M;
+  const unsigned grid = grid_for_work(items);
+  k_layer_reg<T, ADJ, M><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, ADJ ? (V*)lam->d : nullptr, items, ins, u, g,
+                                                             g_lw.part);
+  if (!ADJ) {
+    IQSI_TRY(hipGetLastError());
+    return IQSH_OK;
+  }
+  return finish_pass(grid, acc);
+}
+
+template <typename T, bool ADJ>
+int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm, double* grad)
+{
+  using V = typename Vec<T>::t;
+  const Mat2<T> u{{(T)um[0], (T)um[1]}, {(T)um[2], (T)um[3]}, {(T)um[4], (T)um[5]}, {(T)um[6], (T)um[7]}};
+  Mat2<double> g{{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  if (ADJ) g = Mat2<double>{{gm[0], gm[1]}, {gm[2], gm[3]}, {gm[4], gm[5]}, {gm[6], gm[7]}};
+  double acc = 0;
+  const unsigned n = h->n;
+  const int L = (int)std::min<unsigned>(n, kLowQ);
+  {
+    const uint64_t rows = h->size() / kLowP;
+    const unsigned grid = grid_for_work(rows);
+    k_layer_low<T, ADJ><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, ADJ ? (V*)lam->d : nullptr, h->size(), L, u, g,
+                                                           g_lw.part);
+    if (ADJ) {
+      if (int rc = finish_pass(grid, &acc)) return rc;
+    } else {
+      IQSI_TRY(hipGetLastError());
+    }
+  }
+  for (unsigned p = kLowQ; p < n; p += kRegQ) {
+    const unsigned m = std::min<unsigned>(kRegQ, n - p);
+    int rc = IQSH_OK;
+    if (m == 4) rc = reg_pass<T, ADJ, 4>(h, lam, p, u, g, &acc);
+    else if (m == 3) rc = reg_pass<T, ADJ, 3>(h, lam, p, u, g, &acc);
+    else if (m == 2) rc = reg_pass<T, ADJ, 2>(h, lam, p, u, g, &acc);
+    else rc = reg_pass<T, ADJ, 1>(h, lam, p, u, g, &acc);
+    if (rc) return rc;
+  }
+  if (ADJ) *grad = 2.0 * acc;
+  return IQSH_OK;
+}
+
+// The lane pass needs whole rows of kLB * kLowP amplitudes with lanes spanning >= 64.
+constexpr unsigned kMinLayerQubits = 10;
+
+}  // namespace
+
+extern "C" {
+
+int iqsh_apply_1q_all(iqsh_handle h, const double m[8])
+{
+  if (!h || !h->d) return iqsi::fail(IQSH_ERR_ARG, "null or released state handle");
+  if (!m) return iqsi::fail(IQSH_ERR_ARG, "null matrix");
+  if (h->n < kMinLayerQubits) return iqsi::fail(IQSH_ERR_NOTSUP, "layer kernels need >= %u qubits", kMinLayerQubits);
+  std::lock_guard<std::mutex> lk(g_lm);
+  return h->dtype == IQSH_C128 ? do_layer<double, false>(h, nullptr, m, nullptr, nullptr)
+                               : do_layer<float, false>(h, nullptr, m, nullptr, nullptr);
+}
+
+int iqsh_adjoint_1q_all(iqsh_handle psi, iqsh_handle lam, const double Uinv[8], const double* G, double* grad)
+{
+  if (!psi || !psi->d || !lam || !lam->d) return iqsi::fail(IQSH_ERR_ARG, "null or released state handle");
+  if (psi->n != lam->n || psi->dtype != lam->dtype) return iqsi::fail(IQSH_ERR_ARG, "registers differ in size or precision");
+  if (!Uinv || !G || !grad) return iqsi::fail(IQSH_ERR_ARG, "null argument");
+  if (psi->n < kMinLayerQubits) return iqsi::fail(IQSH_ERR_NOTSUP, "layer kernels need >= %u qubits", kMinLayerQubits);
+  std::lock_guard<std::mutex> lk(g_lm);
+  if (int rc = part_buf()) return rc;
+  return psi->dtype == IQSH_C128 ? do_layer<double, true>(psi, lam, Uinv, G, grad)
+                                 : do_layer<float, true>(psi, lam, Uinv, G, grad);
+}
+
+}  // extern "C"

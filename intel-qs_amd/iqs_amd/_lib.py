@@ -78,6 +78,8 @@ def _sig(L):
         "iqsh_qaoa_cost_apply": [P, P, U, P, P, P, DP],
         "iqsh_qaoa_phase_adjoint": [P, P, U, P, P, P, D, DP],
         "iqsh_adjoint_1q": [P, P, U, P, P, DP],
+        "iqsh_apply_1q_all": [P, P],
+        "iqsh_adjoint_1q_all": [P, P, P, P, DP],
         "iqsh_hea_variants": [I, C.POINTER(I)],
         "iqsh_hea_variant_info": [I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(I)],
         "iqsh_hea_create": [I, I, P, P, P, C.POINTER(P)],

@@ -101,6 +101,11 @@ class StateVector:
         a = _cmat(m, 4)
         call("iqsh_apply_1q", self._h, pos, a.ctypes.data)
 
+    def apply_1q_all(self, m) -> None:
+        """The same 2x2 matrix on every qubit (a few fused passes; >= 10 qubits)."""
+        a = _cmat(m, 4)
+        call("iqsh_apply_1q_all", self._h, a.ctypes.data)
+
     def apply_1q_spec(self, pos: int, spec: int, angle: float = 0.0) -> None:
         call("iqsh_apply_1q_spec", self._h, pos, spec, float(angle))
 
@@ -216,6 +221,14 @@ class StateVector:
         ne, u, v, w, _keep = _edges(edges)
         r = C.c_double()
         call("iqsh_qaoa_phase_adjoint", self._h, lam._h, ne, u, v, w, float(gamma), C.byref(r))
+        return r.value
+
+    def adjoint_1q_all(self, lam: "StateVector", uinv, G) -> float:
+        """Backward step through a layer of the same gate on every qubit: returns
+        sum_q 2 Im <lam| G_q |psi>, then applies uinv to every qubit of both states."""
+        a, g = _cmat(uinv, 4), _cmat(G, 4)
+        r = C.c_double()
+        call("iqsh_adjoint_1q_all", self._h, lam._h, a.ctypes.data, g.ctypes.data, C.byref(r))
         return r.value
 
     def adjoint_1q(self, lam: "StateVector", pos: int, uinv, G=None) -> Optional[float]:

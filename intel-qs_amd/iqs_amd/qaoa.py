@@ -11,8 +11,9 @@ Adjoint method (no counterpart in the reference): after the forward pass psi = p
 lambda = C psi.  Walking the gates backwards, each parameterized gate U = exp(-i theta G)
 contributes dC/dtheta = 2 Im <lambda| G |psi> (psi is the state just after U), then U is
 undone on both vectors.  The mixer's qubit rotations commute with one another and with
-every X_q, so <lambda|X_q|psi> can be taken while the rotations are undone one qubit at a
-time.  Memory: the state, lambda, and nothing else (no checkpoints).
+every X_q, so each <lambda|X_q|psi> can be taken in the fused pass that undoes its qubit's
+rotation (six, then four qubits per pass: iqsh_adjoint_1q_all).  Memory: the state,
+lambda, and nothing else (no checkpoints).
 """
 from __future__ import annotations
 
@@ -27,6 +28,9 @@ from .circuits import SplitMix64, random_regular_graph
 from .engine import StateVector
 
 X = np.array([[0, 1], [1, 0]], dtype=np.complex128)
+# Registers from this size on apply the mixer with the fused layer kernels
+# (iqsh_apply_1q_all / iqsh_adjoint_1q_all); smaller ones qubit by qubit.
+LAYER_MIN_QUBITS = 10
 
 
 @dataclass
@@ -54,6 +58,9 @@ class MaxCutQaoa:
 
     def _mixer(self, psi: StateVector, beta: float) -> None:
         m = G.rx(2.0 * beta)
+        if self.num_qubits >= LAYER_MIN_QUBITS:
+            psi.apply_1q_all(m)  # RX on every qubit in fused passes
+            return
         for q in range(self.num_qubits):
             psi.apply_1q(q, m)
 
@@ -93,10 +100,13 @@ class MaxCutQaoa:
         dg, db = np.zeros(p), np.zeros(p)
         for l in reversed(range(p)):
             uinv = G.rx(-2.0 * betas[l])
-            s = 0.0
-            for q in reversed(range(self.num_qubits)):
-                s += psi.adjoint_1q(lam, q, uinv, X)
-            db[l] = s
+            if self.num_qubits >= LAYER_MIN_QUBITS:
+                db[l] = psi.adjoint_1q_all(lam, uinv, X)
+            else:
+                s = 0.0
+                for q in reversed(range(self.num_qubits)):
+                    s += psi.adjoint_1q(lam, q, uinv, X)
+                db[l] = s
             dg[l] = psi.qaoa_phase_adjoint(lam, self.edges, gammas[l])
         return cost, dg, db
 

@@ -193,6 +193,19 @@ int iqsh_apply_1q_all(iqsh_handle h, const double m[8]);
  * lam := Uinv^(x)n lam.  n >= 10. */
 int iqsh_adjoint_1q_all(iqsh_handle psi, iqsh_handle lam, const double Uinv[8], const double *G, double *grad);
 
+/* One QAOA layer, psi := mixer^(x)n exp(-i gamma C) psi with C the cut of the edge list (as
+ * iqsh_qaoa_phase), the phase fused into the mixer's first pass: the reference's
+ * ImplementQaoaLayerBasedOnCostFunction followed by the per-qubit mixer loop
+ * (src/qaoa_features.cpp:248-264, unit_test/include/qaoa_features_test.hpp:75-76).  n >= 10. */
+int iqsh_qaoa_layer(iqsh_handle h, unsigned ne, const unsigned *u, const unsigned *v, const double *w, double gamma,
+                    const double mixer[8]);
+/* Backward step through one QAOA layer for both states: *dmixer := sum_q 2 Im <lam| G_q |psi>
+ * (as iqsh_adjoint_1q_all), the mixer undone with mixer_inv, then *dgamma := 2 Im <lam| C |psi>
+ * and the phase undone (as iqsh_qaoa_phase_adjoint) -- fused into the same passes.  n >= 10. */
+int iqsh_qaoa_layer_adjoint(iqsh_handle psi, iqsh_handle lam, unsigned ne, const unsigned *u, const unsigned *v,
+                            const double *w, double gamma, const double mixer_inv[8], const double *G,
+                            double *dgamma, double *dmixer);
+
 /* ---- batches of hardware-efficient-ansatz circuits ------------------------------------
  * No counterpart in the reference; the nearest is its pool of independent states
  * (src/mpi_env.cpp:239-404, one state per MPI rank group).  A batch holds `count`

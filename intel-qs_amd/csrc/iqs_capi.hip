@@ -785,6 +785,11 @@ int device_cus()
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   return cus;
 }
+int build_cut_spec(iqsh_handle h, unsigned ne, const unsigned* u, const unsigned* v, const double* w, double gamma,
+                   iqsk::CutSpec* cs)
+{
+  return build_cut(h, ne, u, v, w, gamma, cs);
+}
 }  // namespace iqsi
 
 // Definitions take C linkage from their declarations in iqs_hip.h.

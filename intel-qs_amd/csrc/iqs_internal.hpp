@@ -18,6 +18,10 @@ struct iqsh_state {
   size_t esz() const { return dtype == IQSH_C128 ? 16 : 8; }
 };
 
+namespace iqsk {
+struct CutSpec;
+}
+
 namespace iqsi {
 
 // Record a failure for iqsh_last_error() and return `code`.
@@ -26,6 +30,10 @@ int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 hipStream_t stream();
 // Compute units of the current device.
 int device_cus();
+// Edge-parity tables and phase table of a MaxCut cost for register h (at most 64 edges):
+// the phase exp(-i gamma cut) per amplitude.
+int build_cut_spec(iqsh_handle h, unsigned ne, const unsigned* u, const unsigned* v, const double* w, double gamma,
+                   iqsk::CutSpec* cs);
 
 }  // namespace iqsi
 

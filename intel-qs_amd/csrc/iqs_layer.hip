@@ -33,7 +33,7 @@ constexpr int kLB = 256;          // threads per workgroup
 constexpr int kLowQ = 6;          // positions handled by the lane-shuffle pass
 constexpr int kLowP = 4;          // amplitudes per lane in the lane-shuffle pass
 constexpr int kRegQ = 4;          // positions per register pass
-constexpr unsigned kMaxGrid = 2048;
+constexpr unsigned kMaxGrid = 2048;    // the partial buffer holds 2 * kMaxGrid sums
 
 struct LayerWork {
   double* part = nullptr;
@@ -44,7 +44,7 @@ std::mutex g_lm;
 
 int part_buf()
 {
-  if (!g_lw.part && hipMalloc((void**)&g_lw.part, sizeof(double) * kMaxGrid) != hipSuccess)
+  if (!g_lw.part && hipMalloc((void**)&g_lw.part, sizeof(double) * 2 * kMaxGrid) != hipSuccess)
     return iqsi::fail(IQSH_ERR_ALLOC, "layer reduction buffer");
   return IQSH_OK;
 }
@@ -132,6 +132,84 @@ __global__ __launch_bounds__(kLB) void k_layer_low(typename Vec<T>::t* __restric
     }
   }
   if (ADJ) wg_sum_store(acc, part);
+}
+
+// The lane pass with a QAOA phase layer fused in: forward, psi_i *= exp(-i gamma cut_i)
+// before the rotations; adjoint (run as the last pass of the backward step, after every
+// other rotation of the layer is undone), the rotation terms, the undo, then the phase term
+// cut_i Im(conj(lam_i) psi_i) and psi, lam *= exp(+i gamma cut) (cs built with -gamma).
+// part receives two sums per workgroup (rotations, phase).
+template <typename T, bool ADJ>
+__global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict__ psi, typename Vec<T>::t* __restrict__ lam,
+                                                  uint64_t N, int L, Mat2<T> u, Mat2<double> g, CutSpec cs,
+                                                  double* __restrict__ part)
+{
+  using V = typename Vec<T>::t;
+  __shared__ uint64_t tab[8][256];
+  __shared__ double wt[8][256];
+  __shared__ Cx<T> ph[65];
+  build_tables(tab, cs.inc);
+  if (cs.weighted) build_weight_tables(wt, cs);
+  for (unsigned c = threadIdx.x; c <= cs.ne && c < 65; c += kLB) ph[c] = Cx<T>{(T)cs.ph_re[c], (T)cs.ph_im[c]};
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  double acc = 0, accg = 0;
+  const uint64_t rows = N / (kLB * kLowP), G = gridDim.x;
+  const uint64_t lb = (G & 7) ? blockIdx.x : xcd_block(blockIdx.x, G);
+  const uint64_t chunk = (rows + G - 1) / G;
+  const uint64_t b0 = lb * chunk, b1 = (b0 + chunk < rows) ? b0 + chunk : rows;
+  for (uint64_t rb = b0; rb < b1; ++rb) {
+    const uint64_t r0 = rb * (kLB * kLowP) + threadIdx.x;
+    V x[kLowP], y[kLowP];
+    Cx<T> f[kLowP];
+    double cut[kLowP];
+#pragma unroll
+    for (int j = 0; j < kLowP; ++j) {
+      x[j] = ldg(&psi[r0 + (uint64_t)j * kLB]);
+      if (ADJ) y[j] = ldg(&lam[r0 + (uint64_t)j * kLB]);
+      int ic;
+      cut[j] = cut_value(tab, wt, cs, r0 + (uint64_t)j * kLB, &ic);
+      if (!cs.weighted) {
+        f[j] = ph[ic];
+      } else {
+        double sn, co;
+        sincos(cs.gamma * cut[j], &sn, &co);
+        f[j] = Cx<T>{(T)co, (T)-sn};
+      }
+      if (!ADJ) x[j] = cmul(f[j], x[j]);
+    }
+    for (int q = 0; q < L; ++q) {
+      const int msk = 1 << q;
+      const bool hi = (lane >> q) & 1;
+#pragma unroll
+      for (int j = 0; j < kLowP; ++j) {
+        const V xp = shfl_v(x[j], msk);
+        const V x0 = hi ? xp : x[j], x1 = hi ? x[j] : xp;
+        if (ADJ) {
+          acc += hi ? im_term(y[j], g.m10, x0, g.m11, x1) : im_term(y[j], g.m00, x0, g.m01, x1);
+          const V yp = shfl_v(y[j], msk);
+          const V y0 = hi ? yp : y[j], y1 = hi ? y[j] : yp;
+          y[j] = hi ? mat_row(u.m10, y0, u.m11, y1) : mat_row(u.m00, y0, u.m01, y1);
+        }
+        x[j] = hi ? mat_row(u.m10, x0, u.m11, x1) : mat_row(u.m00, x0, u.m01, x1);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kLowP; ++j) {
+      if (ADJ) {
+        accg += cut[j] * ((double)y[j].x * x[j].y - (double)y[j].y * x[j].x);
+        x[j] = cmul(f[j], x[j]);
+        y[j] = cmul(f[j], y[j]);
+        stg(y[j], &lam[r0 + (uint64_t)j * kLB]);
+      }
+      stg(x[j], &psi[r0 + (uint64_t)j * kLB]);
+    }
+  }
+  if (ADJ) {
+    wg_sum_store(acc, part);
+    __syncthreads();
+    wg_sum_store(accg, part + G);
+  }
 }
 
 // Register pass over M positions ins (all >= 6): each thread holds the 2^M amplitudes of a
@@ -237,26 +315,45 @@ int reg_pass(iqsh_handle h, iqsh_handle lam, unsigned p0, const Mat2<T>& u, cons
 }
 
 template <typename T, bool ADJ>
-int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm, double* grad)
+int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm, const CutSpec* cs, double* grad,
+             double* grad_phase)
 {
   using V = typename Vec<T>::t;
   const Mat2<T> u{{(T)um[0], (T)um[1]}, {(T)um[2], (T)um[3]}, {(T)um[4], (T)um[5]}, {(T)um[6], (T)um[7]}};
   Mat2<double> g{{0, 0}, {0, 0}, {0, 0}, {0, 0}};
   if (ADJ) g = Mat2<double>{{gm[0], gm[1]}, {gm[2], gm[3]}, {gm[4], gm[5]}, {gm[6], gm[7]}};
-  double acc = 0;
+  double acc = 0, accg = 0;
   const unsigned n = h->n;
   const int L = (int)std::min<unsigned>(n, kLowQ);
-  {
+  // Forward: the lane pass first (it carries the phase layer, applied before the mixer);
+  // adjoint: last (the phase is undone after the whole mixer).
+  auto low_pass = [&]() -> int {
     const uint64_t rows = h->size() / kLowP;
     const unsigned grid = grid_for_work(rows);
-    k_layer_low<T, ADJ><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, ADJ ? (V*)lam->d : nullptr, h->size(), L, u, g,
-                                                           g_lw.part);
-    if (ADJ) {
-      if (int rc = finish_pass(grid, &acc)) return rc;
-    } else {
+    if (cs)
+      k_qaoa_low<T, ADJ><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, ADJ ? (V*)lam->d : nullptr, h->size(), L, u, g,
+                                                            *cs, g_lw.part);
+    else
+      k_layer_low<T, ADJ><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, ADJ ? (V*)lam->d : nullptr, h->size(), L, u, g,
+                                                             g_lw.part);
+    if (!ADJ) {
       IQSI_TRY(hipGetLastError());
+      return IQSH_OK;
     }
-  }
+    if (int rc = finish_pass(grid, &acc)) return rc;
+    if (cs) {
+      // The phase sums follow the rotation sums in the partial buffer.
+      double s = 0;
+      IQSI_TRY(hipMemcpyAsync(g_lw.host.data(), g_lw.part + grid, sizeof(double) * grid, hipMemcpyDeviceToHost,
+                              iqsi::stream()));
+      IQSI_TRY(hipStreamSynchronize(iqsi::stream()));
+      for (unsigned b = 0; b < grid; ++b) s += g_lw.host[b];
+      accg += s;
+    }
+    return IQSH_OK;
+  };
+  if (!ADJ)
+    if (int rc = low_pass()) return rc;
   // Positions above the lane pass in ceil(rest / 4) passes of as equal a size as possible
   // (28 qubits: 4, 4, 4, 4, 3, 3 -- a pass of two positions streams ~20% slower).
   const unsigned rest = n > (unsigned)kLowQ ? n - kLowQ : 0;
@@ -271,7 +368,10 @@ int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm,
     if (rc) return rc;
     p += m;
   }
+  if (ADJ)
+    if (int rc = low_pass()) return rc;
   if (ADJ) *grad = 2.0 * acc;
+  if (ADJ && grad_phase) *grad_phase = 2.0 * accg;
   return IQSH_OK;
 }
 
@@ -288,8 +388,38 @@ int iqsh_apply_1q_all(iqsh_handle h, const double m[8])
   if (!m) return iqsi::fail(IQSH_ERR_ARG, "null matrix");
   if (h->n < kMinLayerQubits) return iqsi::fail(IQSH_ERR_NOTSUP, "layer kernels need >= %u qubits", kMinLayerQubits);
   std::lock_guard<std::mutex> lk(g_lm);
-  return h->dtype == IQSH_C128 ? do_layer<double, false>(h, nullptr, m, nullptr, nullptr)
-                               : do_layer<float, false>(h, nullptr, m, nullptr, nullptr);
+  return h->dtype == IQSH_C128 ? do_layer<double, false>(h, nullptr, m, nullptr, nullptr, nullptr, nullptr)
+                               : do_layer<float, false>(h, nullptr, m, nullptr, nullptr, nullptr, nullptr);
+}
+
+int iqsh_qaoa_layer(iqsh_handle h, unsigned ne, const unsigned* u, const unsigned* v, const double* w, double gamma,
+                    const double mixer[8])
+{
+  if (!h || !h->d) return iqsi::fail(IQSH_ERR_ARG, "null or released state handle");
+  if (!mixer || (ne && (!u || !v))) return iqsi::fail(IQSH_ERR_ARG, "null argument");
+  if (h->n < kMinLayerQubits) return iqsi::fail(IQSH_ERR_NOTSUP, "layer kernels need >= %u qubits", kMinLayerQubits);
+  CutSpec cs;
+  if (int rc = iqsi::build_cut_spec(h, ne, u, v, w, gamma, &cs)) return rc;
+  std::lock_guard<std::mutex> lk(g_lm);
+  return h->dtype == IQSH_C128 ? do_layer<double, false>(h, nullptr, mixer, nullptr, &cs, nullptr, nullptr)
+                               : do_layer<float, false>(h, nullptr, mixer, nullptr, &cs, nullptr, nullptr);
+}
+
+int iqsh_qaoa_layer_adjoint(iqsh_handle psi, iqsh_handle lam, unsigned ne, const unsigned* u, const unsigned* v,
+                            const double* w, double gamma, const double mixer_inv[8], const double* G,
+                            double* dgamma, double* dmixer)
+{
+  if (!psi || !psi->d || !lam || !lam->d) return iqsi::fail(IQSH_ERR_ARG, "null or released state handle");
+  if (psi->n != lam->n || psi->dtype != lam->dtype) return iqsi::fail(IQSH_ERR_ARG, "registers differ in size or precision");
+  if (!mixer_inv || !G || !dgamma || !dmixer || (ne && (!u || !v))) return iqsi::fail(IQSH_ERR_ARG, "null argument");
+  if (psi->n < kMinLayerQubits) return iqsi::fail(IQSH_ERR_NOTSUP, "layer kernels need >= %u qubits", kMinLayerQubits);
+  CutSpec cs;
+  // Un-applying exp(-i gamma C) is applying exp(-i (-gamma) C).
+  if (int rc = iqsi::build_cut_spec(psi, ne, u, v, w, -gamma, &cs)) return rc;
+  std::lock_guard<std::mutex> lk(g_lm);
+  if (int rc = part_buf()) return rc;
+  return psi->dtype == IQSH_C128 ? do_layer<double, true>(psi, lam, mixer_inv, G, &cs, dmixer, dgamma)
+                                 : do_layer<float, true>(psi, lam, mixer_inv, G, &cs, dmixer, dgamma);
 }
 
 int iqsh_adjoint_1q_all(iqsh_handle psi, iqsh_handle lam, const double Uinv[8], const double* G, double* grad)
@@ -300,8 +430,8 @@ int iqsh_adjoint_1q_all(iqsh_handle psi, iqsh_handle lam, const double Uinv[8], 
   if (psi->n < kMinLayerQubits) return iqsi::fail(IQSH_ERR_NOTSUP, "layer kernels need >= %u qubits", kMinLayerQubits);
   std::lock_guard<std::mutex> lk(g_lm);
   if (int rc = part_buf()) return rc;
-  return psi->dtype == IQSH_C128 ? do_layer<double, true>(psi, lam, Uinv, G, grad)
-                                 : do_layer<float, true>(psi, lam, Uinv, G, grad);
+  return psi->dtype == IQSH_C128 ? do_layer<double, true>(psi, lam, Uinv, G, nullptr, grad, nullptr)
+                                 : do_layer<float, true>(psi, lam, Uinv, G, nullptr, grad, nullptr);
 }
 
 }  // extern "C"

@@ -80,6 +80,8 @@ def _sig(L):
         "iqsh_adjoint_1q": [P, P, U, P, P, DP],
         "iqsh_apply_1q_all": [P, P],
         "iqsh_adjoint_1q_all": [P, P, P, P, DP],
+        "iqsh_qaoa_layer": [P, U, P, P, P, D, P],
+        "iqsh_qaoa_layer_adjoint": [P, P, U, P, P, P, D, P, P, DP, DP],
         "iqsh_hea_variants": [I, C.POINTER(I)],
         "iqsh_hea_variant_info": [I, I, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(I)],
         "iqsh_hea_create": [I, I, P, P, P, C.POINTER(P)],

@@ -223,6 +223,21 @@ class StateVector:
         call("iqsh_qaoa_phase_adjoint", self._h, lam._h, ne, u, v, w, float(gamma), C.byref(r))
         return r.value
 
+    def qaoa_layer(self, edges, gamma: float, mixer) -> None:
+        """exp(-i gamma C) then the same mixer on every qubit, fused (>= 10 qubits)."""
+        ne, u, v, w, _keep = _edges(edges)
+        a = _cmat(mixer, 4)
+        call("iqsh_qaoa_layer", self._h, ne, u, v, w, float(gamma), a.ctypes.data)
+
+    def qaoa_layer_adjoint(self, lam: "StateVector", edges, gamma: float, mixer_inv, G) -> Tuple[float, float]:
+        """Backward step through one QAOA layer: (d/dgamma, sum_q 2 Im <lam|G_q|psi>)."""
+        ne, u, v, w, _keep = _edges(edges)
+        a, g = _cmat(mixer_inv, 4), _cmat(G, 4)
+        dg, dm = C.c_double(), C.c_double()
+        call("iqsh_qaoa_layer_adjoint", self._h, lam._h, ne, u, v, w, float(gamma), a.ctypes.data, g.ctypes.data,
+             C.byref(dg), C.byref(dm))
+        return dg.value, dm.value
+
     def adjoint_1q_all(self, lam: "StateVector", uinv, G) -> float:
         """Backward step through a layer of the same gate on every qubit: returns
         sum_q 2 Im <lam| G_q |psi>, then applies uinv to every qubit of both states."""

@@ -72,11 +72,16 @@ class MaxCutQaoa:
         (an edge phase or a qubit RX) used by the parameter-shift rule."""
         psi = self._state("_psi")
         self._prepare(psi)
+        fused = self.num_qubits >= LAYER_MIN_QUBITS
         for l, (g, b) in enumerate(zip(gammas, betas)):
-            psi.qaoa_phase(self.edges, g)
             if shift and shift[0] == "edge" and shift[1] == l:
+                # diagonal phases commute: the extra edge phase may precede the layer
                 psi.qaoa_phase([self.edges[shift[2]]], shift[3])
-            self._mixer(psi, b)
+            if fused:
+                psi.qaoa_layer(self.edges, g, G.rx(2.0 * b))  # phase fused into the mixer
+            else:
+                psi.qaoa_phase(self.edges, g)
+                self._mixer(psi, b)
             if shift and shift[0] == "qubit" and shift[1] == l:
                 psi.apply_1q(shift[2], G.rx(2.0 * shift[3]))
             if self.renorm_every and (l + 1) % self.renorm_every == 0:
@@ -101,12 +106,12 @@ class MaxCutQaoa:
         for l in reversed(range(p)):
             uinv = G.rx(-2.0 * betas[l])
             if self.num_qubits >= LAYER_MIN_QUBITS:
-                db[l] = psi.adjoint_1q_all(lam, uinv, X)
-            else:
-                s = 0.0
-                for q in reversed(range(self.num_qubits)):
-                    s += psi.adjoint_1q(lam, q, uinv, X)
-                db[l] = s
+                dg[l], db[l] = psi.qaoa_layer_adjoint(lam, self.edges, gammas[l], uinv, X)
+                continue
+            s = 0.0
+            for q in reversed(range(self.num_qubits)):
+                s += psi.adjoint_1q(lam, q, uinv, X)
+            db[l] = s
             dg[l] = psi.qaoa_phase_adjoint(lam, self.edges, gammas[l])
         return cost, dg, db
 

@@ -81,3 +81,46 @@ def test_layer_needs_ten_qubits():
     sv = StateVector(9, "c128")
     with pytest.raises(IqsError):
         sv.apply_1q_all(np.eye(2))
+
+
+def _graph(n, seed, weighted):
+    r = np.random.default_rng(seed)
+    edges = [(i, (i + 1) % n, float(r.uniform(0.3, 1.9)) if weighted else 1.0) for i in range(n)]
+    edges += [(0, n // 2, 0.7 if weighted else 1.0), (1, n - 2, 1.3 if weighted else 1.0)]
+    return edges
+
+
+@pytest.mark.parametrize("dt", ["c128", "c64"])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_qaoa_layer_fused_equals_unfused(dt, weighted):
+    """iqsh_qaoa_layer = iqsh_qaoa_phase then the mixer on every qubit; the fused adjoint =
+    iqsh_adjoint_1q_all then iqsh_qaoa_phase_adjoint (gradients and states); the forward
+    layer also against the oracle (cost diagonal from the edge list, 1q kernel per qubit)."""
+    from iqs_amd import gates as Gt
+    n = 12
+    edges = _graph(n, 3, weighted)
+    gamma, beta = 0.83, 0.41
+    psi = _state(n, dt, 41)
+    a, b = _gpu(psi, dt), _gpu(psi, dt)
+    a.qaoa_layer(edges, gamma, Gt.rx(2 * beta))
+    b.qaoa_phase(edges, gamma)
+    b.apply_1q_all(Gt.rx(2 * beta))
+    assert np.abs(a.download() - b.download()).max() < TOL[dt]
+    h = psi.astype(np.complex128).copy()
+    x = np.arange(1 << n)
+    cut = sum(w * (((x >> u) ^ (x >> v)) & 1) for u, v, w in edges)
+    h *= np.exp(-1j * gamma * cut)
+    for q in range(n):
+        oracle.apply_1q(h, q, Gt.rx(2 * beta))
+    assert np.abs(a.download() - h).max() < (1e-12 if dt == "c128" else 2e-5)
+    lam = _state(n, dt, 42)
+    X = np.array([[0, 1], [1, 0]])
+    p1, l1 = _gpu(psi, dt), _gpu(lam, dt)
+    dg1, db1 = p1.qaoa_layer_adjoint(l1, edges, gamma, Gt.rx(-2 * beta), X)
+    p2, l2 = _gpu(psi, dt), _gpu(lam, dt)
+    db2 = p2.adjoint_1q_all(l2, Gt.rx(-2 * beta), X)
+    dg2 = p2.qaoa_phase_adjoint(l2, edges, gamma)
+    tol = 1e-11 if dt == "c128" else 2e-5
+    assert abs(db1 - db2) < tol * max(1, abs(db2)) and abs(dg1 - dg2) < tol * max(1, abs(dg2))
+    assert np.abs(p1.download() - p2.download()).max() < TOL[dt]
+    assert np.abs(l1.download() - l2.download()).max() < TOL[dt]

@@ -310,6 +310,7 @@ TM2x2<Type> ChannelBranchMatrix(CM4x4<Type>& chi, RandomNumberGenerator<typename
 
 }  // namespace iqs
 
-// Derived register that inserts noise gates automatically (the reference includes it here
-// too, include/qureg.hpp:432-433).
+// Derived registers that count gates and depth, and that insert noise gates automatically
+// (the reference includes both here too, include/qureg.hpp:429-433).
+#include "QubitRegisterMetric.hpp"
 #include "NoisyQureg.hpp"

@@ -354,6 +354,24 @@ PYBIND11_MODULE(intelqs_py, m)
                " qubits in GPU memory>";
       });
 
+  // Extension: the C++ QubitRegisterMetric (include/QubitRegisterMetric.hpp).
+  using QM = iqs::QubitRegisterMetric<ComplexDP>;
+  py::class_<QM, QubitRegister<ComplexDP>>(m, "QubitRegisterMetric", py::buffer_protocol(), py::dynamic_attr())
+      .def(py::init<int>())
+      .def("GetTotalQubitGateCount", &QM::GetTotalQubitGateCount)
+      .def("GetOneQubitGateCount", &QM::GetOneQubitGateCount)
+      .def("GetTwoQubitGateCount", &QM::GetTwoQubitGateCount)
+      .def("GetParallelDepth", &QM::GetParallelDepth)
+      .def("ApplyHadamard", &QM::ApplyHadamard)
+      .def("ApplyRotationX", &QM::ApplyRotationX)
+      .def("ApplyRotationY", &QM::ApplyRotationY)
+      .def("ApplyRotationZ", &QM::ApplyRotationZ)
+      .def("ApplyCPauliX", &QM::ApplyCPauliX)
+      .def("ApplyControlled1QubitGate",
+           [](QM& a, int q1, int q2, py::array_t<ComplexDP, py::array::c_style | py::array::forcecast> m) {
+             a.ApplyControlled1QubitGate(q1, q2, tm2<ComplexDP>(m, "Input shape is not 2x2."));
+           });
+
   // Extension: the C++ NoisyQureg (include/NoisyQureg.hpp) for Python callers.
   using NQ = iqs::NoisyQureg<ComplexDP>;
   py::class_<NQ, QubitRegister<ComplexDP>>(m, "NoisyQureg", py::buffer_protocol(), py::dynamic_attr())

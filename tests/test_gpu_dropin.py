@@ -248,3 +248,24 @@ def test_sp_register_and_errors():
         psi.ApplyHadamard(10)
     with pytest.raises(RuntimeError, match="2x2"):
         psi.Apply1QubitGate(0, np.eye(3, dtype=np.complex64))
+
+
+def test_qubit_register_metric_counts_and_depth():
+    """QubitRegisterMetric (include/QubitRegisterMetric.hpp of the reference): 1q gates add a
+    layer on their qubit, 2q gates lift both qubits one layer above the deeper one; the
+    state is the plain register's."""
+    m = iqs.QubitRegisterMetric(5)
+    m.Initialize("base", 0)
+    p = iqs.QubitRegister(5, "base", 0, 0)
+    ops = [("ApplyHadamard", 0), ("ApplyHadamard", 1), ("ApplyRotationX", 1, 0.3), ("ApplyCPauliX", 0, 1),
+           ("ApplyRotationZ", 4, 1.1), ("ApplyCPauliX", 1, 2), ("ApplyRotationY", 3, 0.5)]
+    for op in ops:
+        getattr(m, op[0])(*op[1:])
+        getattr(p, op[0])(*op[1:])
+    G = np.array([[0, 1j], [1j, 0]])
+    m.ApplyControlled1QubitGate(2, 3, G)
+    p.ApplyControlled1QubitGate(2, 3, G)
+    # depths: q0 1, q1 2 -> CX(0,1): 3,3 -> CX(1,2): 4,4 -> q3 1 -> C(2,3): 5,5
+    assert (m.GetOneQubitGateCount(), m.GetTwoQubitGateCount(), m.GetTotalQubitGateCount()) == (5, 3, 8)
+    assert m.GetParallelDepth() == 5
+    assert np.array_equal(np.array(m, copy=True), np.array(p, copy=True))

@@ -225,6 +225,50 @@ def qaoa_param_shift_on_reference(torch, n: int = 16, p: int = 8):
             "max_abs_diff_gpu_adjoint_vs_cpu_param_shift": diff}
 
 
+def hea_sample_on_reference(torch, count: int = 24, max_qubits: int = 20, budget_s: float = 30.0):
+    """The batched-HEA workload on the host: the first instances of the 512-instance batch
+    (seed 20240606) with at most max_qubits qubits, forward pass and <sum_q Z_q>, through the
+    compiled reference one instance after another (OpenMP inside each gate), against the
+    same instances' costs from the GPU batch engine in complex128."""
+    import numpy as np
+    from iqs_amd.batch import HeaBatch, instance_circuit, param_offsets, random_batch
+
+    drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if not os.path.exists(drv):
+        return None
+    nq, dp, ang = random_batch()
+    off = param_offsets(nq, dp)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    picked, ref_cost, cpu_s = [], [], 0.0
+    with tempfile.TemporaryDirectory() as d:
+        for i in range(len(nq)):
+            if len(picked) >= count or cpu_s > budget_s:
+                break
+            if nq[i] > max_qubits:
+                continue
+            c = instance_circuit(nq[i], dp[i], ang[off[i]:off[i + 1]])
+            cf, rf = os.path.join(d, "c.txt"), os.path.join(d, "r.txt")
+            c.write(cf)
+            r = subprocess.run([drv, cf, "--results", rf, "--time"], capture_output=True, text=True, timeout=600,
+                               env=dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread",
+                                        OMP_PLACES="cores"))
+            if r.returncode != 0:
+                return None
+            cpu_s += sum(float(l.split()[3]) for l in r.stderr.splitlines() if l.startswith("T "))
+            z = [float(x) for l in open(rf) if l.startswith("R ZAll") for x in l.split()[2:]]
+            picked.append(i)
+            ref_cost.append(sum(z))
+    b = HeaBatch([nq[i] for i in picked], [dp[i] for i in picked], "c128")
+    sample_angles = np.concatenate([ang[off[i]:off[i + 1]] for i in picked])
+    gpu_cost = b.evaluate(sample_angles, grad=False)
+    b.close()
+    return {"workload": f"first {len(picked)} instances with <= {max_qubits} qubits of the 512-instance HEA batch "
+                        f"(seed 20240606), forward pass and <sum Z>",
+            "instances": len(picked), "qubits": [int(min(nq[i] for i in picked)), int(max(nq[i] for i in picked))],
+            "cpu_gate_s": cpu_s, "cpu_threads": threads, "cpu_instances_per_s": len(picked) / cpu_s,
+            "max_abs_cost_diff_gpu_c128_vs_reference": float(np.max(np.abs(np.array(gpu_cost) - np.array(ref_cost))))}
+
+
 def qaoa_measure(n: int, p: int, full: bool, torch):
     """28-qubit QAOA MaxCut gradient: adjoint vs parameter-shift on the same GPU."""
     import numpy as np
@@ -534,6 +578,7 @@ def main() -> int:
     if cpu is not None:
         cpu["config1_20q"] = config1_against_reference(torch)
         cpu["qaoa_16q_param_shift"] = None if args.no_qaoa else qaoa_param_shift_on_reference(torch)
+        cpu["hea_batch_sample"] = None if args.no_batch else hea_sample_on_reference(torch)
     log("CPU baseline done")
 
     out = {

@@ -327,6 +327,12 @@ int iqsh_pool_init(iqsh_handle h, unsigned state_qubits, int style, uint64_t ind
  * differ from state to state). */
 int iqsh_pool_apply_1q_each(iqsh_handle h, unsigned state_qubits, unsigned pos, const double *mats,
                             uint64_t nstates);
+/* A different dense 4x4 matrix per state on positions (ph, pl) < state_qubits, row index
+ * 2*b(ph) + b(pl) as iqsh_apply_2q: mats holds nstates row-major complex 4x4 matrices
+ * (32 doubles each).  What the reference does with one Apply2QubitGate per rank group for
+ * the branches of a 2-qubit channel (src/qureg_apply_channel.cpp:66-106). */
+int iqsh_pool_apply_2q_each(iqsh_handle h, unsigned state_qubits, unsigned ph, unsigned pl, const double *mats,
+                            uint64_t nstates);
 /* out[s] = the reduction `kind` of state s (FP64, fixed order), for s < nstates.  Replaces
  * per-rank-group GetProbability / ComputeNorm / ExpectationValue of Z strings
  * (src/qureg_measure.cpp:136-178, src/qureg_utils.cpp:237-262, src/qureg_expectval.cpp:104-375). */

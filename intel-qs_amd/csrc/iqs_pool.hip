@@ -1,9 +1,9 @@
 // Pool of states: the per-state entry points of libiqs_hip.so (include/iqs_hip.h,
 // "pool of states").  A pool is one register whose top positions index its states, so
 // every ordinary gate kernel already applies a common gate to all states in one launch;
-// the kernels here cover what differs from state to state -- initialization, one 2x2
-// matrix per state (noise gates, channel branches), and reductions with one result per
-// state.
+// the kernels here cover what differs from state to state -- initialization, one 2x2 or
+// 4x4 matrix per state (noise gates, 1- and 2-qubit channel branches), and reductions with
+// one result per state.
 //
 // Reductions: a workgroup reduces one chunk of C = min(2^state_qubits, 2^14) contiguous
 // amplitudes (never straddling two states) in FP64 with a fixed lane/wave order; the
@@ -119,6 +119,34 @@ __global__ __launch_bounds__(kPB) void k_pool_1q(typename Vec<T>::t* __restrict_
   }
 }
 
+// Dense 4x4 per state on (ph, pl), rows 2 b(ph) + b(pl): with lo < hi the two positions in
+// ascending order, item k enumerates the indices with both bits clear; s = index >> sbits.
+template <typename T>
+__global__ __launch_bounds__(kPB) void k_pool_2q(typename Vec<T>::t* __restrict__ st, uint64_t items, unsigned lo,
+                                                 unsigned hi, uint64_t dh, uint64_t dl, unsigned sbits,
+                                                 const Mat4<T>* __restrict__ mats, uint64_t nstates)
+{
+  using V = typename Vec<T>::t;
+  for (uint64_t k = (uint64_t)blockIdx.x * kPB + threadIdx.x; k < items; k += (uint64_t)gridDim.x * kPB) {
+    const uint64_t i0 = ins0(ins0(k, lo), hi);
+    const uint64_t s = i0 >> sbits;
+    if (s >= nstates) continue;
+    const Mat4<T>& m = mats[s];
+    const uint64_t off[4] = {0, dl, dh, dh | dl};
+    V x[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) x[c] = st[i0 | off[c]];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      V r = cmad2(m.m[4 * t + 0], x[0], m.m[4 * t + 1], x[1]);
+      const V u = cmad2(m.m[4 * t + 2], x[2], m.m[4 * t + 3], x[3]);
+      r.x += u.x;
+      r.y += u.y;
+      st[i0 | off[t]] = r;
+    }
+  }
+}
+
 // One partial per chunk of 2^clog amplitudes: KIND 0 sum |psi|^2, 1 over bit `arg` set,
 // 2 with the sign (-1)^popcount(i & arg).
 template <typename T, int KIND>
@@ -199,6 +227,24 @@ int do_1q_each(iqsh_handle h, unsigned sbits, unsigned pos, const double* mats, 
   return IQSH_OK;
 }
 
+template <typename T>
+int do_2q_each(iqsh_handle h, unsigned sbits, unsigned ph, unsigned pl, const double* mats, uint64_t nstates)
+{
+  std::vector<Mat4<T>> m(nstates);
+  for (uint64_t s = 0; s < nstates; ++s)
+    for (int e = 0; e < 16; ++e) m[s].m[e] = Cx<T>{(T)mats[32 * s + 2 * e], (T)mats[32 * s + 2 * e + 1]};
+  const size_t bytes = sizeof(Mat4<T>) * nstates;
+  if (int rc = grow(&g_pw.mats, &g_pw.mats_bytes, bytes)) return rc;
+  IQSI_TRY(hipStreamSynchronize(iqsi::stream()));
+  IQSI_TRY(hipMemcpy(g_pw.mats, m.data(), bytes, hipMemcpyHostToDevice));
+  const uint64_t items = h->size() >> 2;
+  k_pool_2q<T><<<grid_of(items), kPB, 0, iqsi::stream()>>>((typename Vec<T>::t*)h->d, items, std::min(ph, pl),
+                                                            std::max(ph, pl), uint64_t(1) << ph, uint64_t(1) << pl,
+                                                            sbits, (const Mat4<T>*)g_pw.mats, nstates);
+  IQSI_TRY(hipGetLastError());
+  return IQSH_OK;
+}
+
 // Launch `nchunks` chunk reductions with NV values each, then sum them per state on the host.
 template <typename Launch>
 int reduce_per_state(unsigned sbits, uint64_t nstates, int nv, double* out, Launch launch)
@@ -270,6 +316,19 @@ int iqsh_pool_apply_1q_each(iqsh_handle h, unsigned state_qubits, unsigned pos, 
   std::lock_guard<std::mutex> lk(g_pm);
   return h->dtype == IQSH_C128 ? do_1q_each<double>(h, state_qubits, pos, mats, nstates)
                                : do_1q_each<float>(h, state_qubits, pos, mats, nstates);
+}
+
+int iqsh_pool_apply_2q_each(iqsh_handle h, unsigned state_qubits, unsigned ph, unsigned pl, const double* mats,
+                            uint64_t nstates)
+{
+  if (int rc = check_pool(h, state_qubits, nstates)) return rc;
+  if (ph >= state_qubits || pl >= state_qubits)
+    return iqsi::fail(IQSH_ERR_ARG, "positions %u, %u outside the state (%u qubits)", ph, pl, state_qubits);
+  if (ph == pl) return iqsi::fail(IQSH_ERR_ARG, "2q positions coincide");
+  if (!mats) return iqsi::fail(IQSH_ERR_ARG, "null matrices");
+  std::lock_guard<std::mutex> lk(g_pm);
+  return h->dtype == IQSH_C128 ? do_2q_each<double>(h, state_qubits, ph, pl, mats, nstates)
+                               : do_2q_each<float>(h, state_qubits, ph, pl, mats, nstates);
 }
 
 int iqsh_pool_reduce(iqsh_handle h, unsigned state_qubits, int kind, uint64_t arg, double* out, uint64_t nstates)

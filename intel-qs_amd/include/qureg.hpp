@@ -307,6 +307,11 @@ TM2x2<Type> NoiseGateMatrix(typename extract_value_type<Type>::value_type T1,
 template <class Type>
 TM2x2<Type> ChannelBranchMatrix(CM4x4<Type>& chi, RandomNumberGenerator<typename extract_value_type<Type>::value_type>& rng,
                                 bool* negative);
+/// The same for a 2-qubit channel (src/qureg_apply_channel.cpp:66-106): a 4x4 in the index
+/// 2 b(qubit1) + b(qubit2).
+template <class Type>
+TM4x4<Type> ChannelBranchMatrix(CM16x16<Type>& chi, RandomNumberGenerator<typename extract_value_type<Type>::value_type>& rng,
+                                bool* negative);
 
 }  // namespace iqs
 

@@ -49,6 +49,8 @@ class StatePool
   void ApplyNoiseGate(unsigned qubit, BaseType duration);
   /// ApplyChannel (src/qureg_apply_channel.cpp:17-58) on every state, each drawing its branch.
   void ApplyChannel(unsigned qubit, CM4x4<Type>& chi);
+  /// 2-qubit ApplyChannel (src/qureg_apply_channel.cpp:66-106) on every state.
+  void ApplyChannel(unsigned qubit1, unsigned qubit2, CM16x16<Type>& chi);
   std::vector<BaseType> GetOverallSignOfChannels() const { return sign_; }
 
   // ---- one value per state ----

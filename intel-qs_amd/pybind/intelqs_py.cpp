@@ -338,6 +338,11 @@ PYBIND11_MODULE(intelqs_py, m)
              CM4x4<ComplexDP> c(chi);
              a.ApplyChannel(qubit, c);
            })
+      .def("ApplyChannel",
+           [](SP& a, unsigned qubit1, unsigned qubit2, const iqs::ChiMatrix<ComplexDP, 16, 32>& chi) {
+             CM16x16<ComplexDP> c(chi);
+             a.ApplyChannel(qubit1, qubit2, c);
+           })
       .def("GetOverallSignOfChannels", &SP::GetOverallSignOfChannels)
       .def("GetProbability", &SP::GetProbability)
       .def("ComputeNorm", &SP::ComputeNorm)

@@ -1320,19 +1320,19 @@ void QubitRegister<Type>::ApplyChannel(const unsigned qubit, CM4x4<Type>& chi)
   Apply1QubitGate(qubit, m);
 }
 
-/// One stochastic step of a 2-qubit channel (qureg_apply_channel.cpp:66-106): the branch
-/// matrix is sum_{a,b} e_{4a+b} sigma_a (qubit1) x sigma_b (qubit2) in the 4x4 index
-/// 2 b(qubit1) + b(qubit2), applied with Apply2QubitGate(qubit1, qubit2, .).  The diagonal
-/// and upper triangle use the reference's expressions; the lower triangle is evaluated from
-/// the same sum (the reference writes it as the conjugate of the upper one, which equals
-/// the sum only for real coefficients) and, as in the 1-qubit case, a negative eigenvalue
-/// flips the overall sign of channels.
+/// Branch of a 2-qubit channel (qureg_apply_channel.cpp:66-106): draw k from the state
+/// stream, then E_k = sum_{a,b} e_{4a+b} sigma_a (qubit1) x sigma_b (qubit2) as a 4x4 in the
+/// index 2 b(qubit1) + b(qubit2).  The diagonal and upper triangle use the reference's
+/// expressions; the lower triangle is evaluated from the same sum (the reference writes it
+/// as the conjugate of the upper one, which equals the sum only for real coefficients) and,
+/// as in the 1-qubit case, a negative eigenvalue is reported so the caller can flip the
+/// overall sign of channels.
 template <class Type>
-void QubitRegister<Type>::ApplyChannel(const unsigned qubit1, const unsigned qubit2, CM16x16<Type>& chi)
+TM4x4<Type> ChannelBranchMatrix(CM16x16<Type>& chi, RandomNumberGenerator<typename extract_value_type<Type>::value_type>& rng,
+                                bool* negative)
 {
-  if (!rng_ptr_) throw std::logic_error("ApplyChannel: no random number generator (SetRngPtr)");
-  BaseType r;
-  rng_ptr_->UniformRandomNumbers(&r, 1, 0, 1, "state");
+  typename extract_value_type<Type>::value_type r;
+  rng.UniformRandomNumbers(&r, 1, 0, 1, "state");
   const unsigned k = channel_branch(chi, r, 16);
   const std::vector<Type> e = chi.GetEigenVector(k);
   const Type I(0., 1.);
@@ -1356,7 +1356,18 @@ void QubitRegister<Type>::ApplyChannel(const unsigned qubit1, const unsigned qub
   m(3, 0) = e[5] - e[10] + I * (e[9] + e[6]);
   m(1, 2) = e[5] + e[10] - I * (e[9] - e[6]);
   m(2, 1) = e[5] + e[10] + I * (e[9] - e[6]);
-  if (std::real(chi.GetEigenValue(k)) < 0) overall_sign_of_channels *= -1;
+  *negative = std::real(chi.GetEigenValue(k)) < 0;
+  return m;
+}
+
+/// One stochastic step of a 2-qubit channel, applied with Apply2QubitGate(qubit1, qubit2, .).
+template <class Type>
+void QubitRegister<Type>::ApplyChannel(const unsigned qubit1, const unsigned qubit2, CM16x16<Type>& chi)
+{
+  if (!rng_ptr_) throw std::logic_error("ApplyChannel: no random number generator (SetRngPtr)");
+  bool negative = false;
+  const TM4x4<Type> m = ChannelBranchMatrix(chi, *rng_ptr_, &negative);
+  if (negative) overall_sign_of_channels *= -1;
   Apply2QubitGate(qubit1, qubit2, m);
 }
 
@@ -1394,5 +1405,7 @@ template TM2x2<ComplexSP> NoiseGateMatrix<ComplexSP>(float, float, float, Random
 template TM2x2<ComplexDP> NoiseGateMatrix<ComplexDP>(double, double, double, RandomNumberGenerator<double>&);
 template TM2x2<ComplexSP> ChannelBranchMatrix<ComplexSP>(CM4x4<ComplexSP>&, RandomNumberGenerator<float>&, bool*);
 template TM2x2<ComplexDP> ChannelBranchMatrix<ComplexDP>(CM4x4<ComplexDP>&, RandomNumberGenerator<double>&, bool*);
+template TM4x4<ComplexSP> ChannelBranchMatrix<ComplexSP>(CM16x16<ComplexSP>&, RandomNumberGenerator<float>&, bool*);
+template TM4x4<ComplexDP> ChannelBranchMatrix<ComplexDP>(CM16x16<ComplexDP>&, RandomNumberGenerator<double>&, bool*);
 
 }  // namespace iqs

@@ -115,6 +115,24 @@ void StatePool<Type>::ApplyChannel(unsigned qubit, CM4x4<Type>& chi)
 }
 
 template <class Type>
+void StatePool<Type>::ApplyChannel(unsigned qubit1, unsigned qubit2, CM16x16<Type>& chi)
+{
+  const unsigned p1 = pos_(qubit1), p2 = pos_(qubit2);
+  if (qubit1 == qubit2) throw std::invalid_argument("StatePool::ApplyChannel: qubits coincide");
+  std::vector<double> mats(32 * std::size_t(ns_));
+  for (unsigned s = 0; s < ns_; ++s) {
+    bool negative = false;
+    const TM4x4<Type> m = ChannelBranchMatrix(chi, gens_[s], &negative);
+    if (negative) sign_[s] *= -1;
+    for (int e = 0; e < 16; ++e) {
+      mats[32 * s + 2 * e] = (double)std::real(m(e / 4, e % 4));
+      mats[32 * s + 2 * e + 1] = (double)std::imag(m(e / 4, e % 4));
+    }
+  }
+  pool_ck(iqsh_pool_apply_2q_each(dev_(), n_, p1, p2, mats.data(), ns_));
+}
+
+template <class Type>
 std::vector<typename StatePool<Type>::BaseType> StatePool<Type>::GetProbability(unsigned qubit)
 {
   const unsigned p = pos_(qubit);

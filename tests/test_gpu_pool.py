@@ -7,8 +7,8 @@ reference's pool, one state per MPI rank group (src/mpi_env.cpp:239-404).
     read-outs per state to 1e-12, the incoherent sum to their sum.
   * Common gates on Register() act on every state as on a lone register (the oracle runs
     the same gate list), padding states stay zero.
-  * Per-state reductions equal numpy on the downloaded states; per-state channel branches
-    equal ApplyChannel on a lone register seeded as that pool state."""
+  * Per-state reductions equal numpy on the downloaded states; per-state 1- and 2-qubit
+    channel branches equal ApplyChannel on a lone register seeded as that pool state."""
 import os
 
 import numpy as np
@@ -141,6 +141,47 @@ def test_per_state_channel_matches_lone_registers():
             reg.ApplyCPauliX(step % n, (step + 2) % n)
         reg.ResetRngPtr()
         assert np.max(np.abs(pool.GetState(s) - np.array(reg, copy=True))) < 1e-12, s
+
+
+@pytest.mark.parametrize("order", ["high_first", "low_first"])
+def test_per_state_2q_channel_matches_lone_registers(order):
+    """2-qubit channel branches (a dense complex chi, so every Pauli pair term is present)
+    drawn per state; both qubit orders pin the 2 b(qubit1) + b(qubit2) row convention of
+    the per-state kernel against Apply2QubitGate.  Three states: the padding state stays 0."""
+    m = _m()
+    n, ns, seed = 6, 3, 57
+    r = np.random.default_rng(12)
+    a = r.normal(size=(16, 16)) + 1j * r.normal(size=(16, 16))
+    chi = a @ a.conj().T
+    chi /= np.trace(chi).real
+    cm = m.CM16x16(chi)
+    cm.SolveEigenSystem()
+    pairs = [(4, 1), (0, 5), (3, 2)] if order == "high_first" else [(1, 4), (5, 0), (2, 3)]
+    pool = m.StatePool(n, ns, "++++", 0)
+    pool.SetSeedStreamPtrs(seed)
+    for step in range(5):
+        q1, q2 = pairs[step % 3]
+        pool.ApplyChannel(q1, q2, cm)
+        pool.Register().ApplyRotationY(step % n, 0.3 + step)
+    signs = pool.GetOverallSignOfChannels()
+    for s in range(ns):
+        reg = m.QubitRegister(n, "++++", 0, 0)
+        rng = m.RandomNumberGenerator()
+        rng.SetSeedStreamPtrs(seed + s)
+        reg.SetRngPtr(rng)
+        for step in range(5):
+            q1, q2 = pairs[step % 3]
+            reg.ApplyChannel(q1, q2, cm)
+            reg.ApplyRotationY(step % n, 0.3 + step)
+        reg.ResetRngPtr()
+        want = np.array(reg, copy=True)
+        assert np.max(np.abs(want)) > 1e-3
+        assert np.max(np.abs(pool.GetState(s) - want)) < 1e-12, s
+        assert signs[s] == reg.GetOverallSignOfChannels()
+    full = np.array(pool.Register(), copy=True)
+    assert np.all(full[ns << n:] == 0)
+    with pytest.raises(ValueError):
+        pool.ApplyChannel(2, 2, cm)
 
 
 def test_pool_errors():

@@ -599,8 +599,8 @@ def main() -> int:
                    "qubits": n, "depth": args.steps, "gates": ngates},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(n),
-                     "kernel": "k_pair<double,1,false,true,1024,2,SUB> (dense 1q update, target >= 6: H, RX, RY; "
-                               "SUB=1 for targets 6-10, SUB=4 above)",
+                     "kernel": "k_pair<double,1,false,true,128,2,4,SL> (dense 1q update, target >= 6: H, RX, RY; "
+                               "sub-ranges 2^SL items apart, SL=10 for targets 6-10, 11 above)",
                      "avg_launch_ms": 1e3 * dense_avg, "launches": len(dense)},
         "per_kind_avg_ms": {k: 1e3 * sum(v) / len(v) for k, v in per_kind.items()},
         "hbm_frac_1q": achieved / HBM_PEAK_GBS,

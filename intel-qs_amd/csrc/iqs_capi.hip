@@ -152,7 +152,14 @@ bool is_c(const double* a, double re, double im) { return a[0] == re && a[1] == 
 // Launchers
 
 // Launch configurations (see the tiling notes in iqs_kernels.hpp).
-constexpr int kTB = 1024, kTP = 2;  // tiled pair / scale / diag kernels
+constexpr int kTB = 1024, kTP = 2;  // tiled scale / diag kernels; pair and quad kernels for complex64
+// Pair and 4x4 kernels, per precision (30q sweep of block size x items per thread x split
+// count): complex128 streams best with 128-thread blocks of 2 items split 4 ways (64-item
+// runs, one wavefront access each): dense 1q 6.15 -> 6.53 TB/s for targets >= 11, controlled
+// 5.88 -> 6.17, dense 2q 5.80 -> 6.18; complex64 keeps 1024 x 2 (smaller blocks are no
+// faster there).
+template <typename T> struct PairTile { static constexpr int B = 1024, P = 2, QP = 1; };
+template <> struct PairTile<double> { static constexpr int B = 128, P = 2, QP = 2; };
 constexpr int kLB = 256, kLP = 2;   // lane-shuffle 1q kernel
 
 // Index bits inside one 128-byte cache line: 3 for complex128, 4 for complex64.
@@ -165,20 +172,20 @@ int launch_pair_m(iqsh_handle h, const Ins<K>& ins, uint64_t mask0, uint64_t mas
 {
   using V = typename Vec<T>::t;
   const uint64_t items = h->size() >> K;
-  // Complex128 with every touched position <= 10: one contiguous range per block (30q sweep:
-  // dense 1q targets 6-10 5.9 -> 6.2 TB/s, controlled pairs below 11 5.8-6.2 -> 6.1-6.4);
-  // split blocks everywhere else.
+  constexpr int TB = PairTile<T>::B, TP = PairTile<T>::P;
+  const dim3 grid = grid2(items / (TB * TP));
   if (items >= kTiledMin && K >= 2 && ins.p[0] < line_bits<T>())
     // Lowest position inside a cache line (a control above it): cached accesses let L2
     // complete the lines that the two halves of each pair share.
-    k_pair<T, K, MOVE, true, kTB, kTP, kSub, kSubLog, false><<<grid2(items / (kTB * kTP)), kTB, 0, g_stream>>>(
-        (V*)h->d, items, ins, mask0, mask1, mat2<T>(m));
+    k_pair<T, K, MOVE, true, TB, TP, kSub, kSubLog, false><<<grid, TB, 0, g_stream>>>((V*)h->d, items, ins, mask0,
+                                                                                           mask1, mat2<T>(m));
   else if (items >= kTiledMin && sizeof(T) == 8 && ins.p[K - 1] <= 10)
-    k_pair<T, K, MOVE, true, kTB, kTP, 1><<<grid2(items / (kTB * kTP)), kTB, 0, g_stream>>>((V*)h->d, items, ins, mask0,
-                                                                                                    mask1, mat2<T>(m));
+    // Complex128 with every touched position <= 10: sub-ranges 2^10 items apart (30q sweep,
+    // dense 1q targets 6-10: 6.44 TB/s at 2^11, 6.57 at 2^10).
+    k_pair<T, K, MOVE, true, TB, TP, kSub, 10><<<grid, TB, 0, g_stream>>>((V*)h->d, items, ins, mask0, mask1,
+                                                                            mat2<T>(m));
   else if (items >= kTiledMin)
-    k_pair<T, K, MOVE, true, kTB, kTP><<<grid2(items / (kTB * kTP)), kTB, 0, g_stream>>>((V*)h->d, items, ins, mask0,
-                                                                                                 mask1, mat2<T>(m));
+    k_pair<T, K, MOVE, true, TB, TP><<<grid, TB, 0, g_stream>>>((V*)h->d, items, ins, mask0, mask1, mat2<T>(m));
   else
     k_pair<T, K, MOVE, false, kSB, kSP><<<grid2(grid_for(items, kSB * kSP)), kSB, 0, g_stream>>>((V*)h->d, items, ins, mask0,
                                                                                           mask1, mat2<T>(m));
@@ -1142,9 +1149,15 @@ int do_2q(iqsh_handle h, unsigned ph, unsigned pl, const double* m)
   if (std::min(ph, pl) < line_bits<T>() && h->size() >= (kTiledMin << 3)) return launch_quad_lane<T>(h, ph, pl, M, false);
   const unsigned ins[2] = {ph, pl};
   const uint64_t items = h->size() >> 2;
-  if (items >= kTiledMin)
+  constexpr int TB = PairTile<T>::B, TP = PairTile<T>::QP;
+  if (items >= kTiledMin && std::max(ph, pl) <= 10)
+    // Both positions <= 10: 1024 x 1 blocks (30q complex128: 6.07-6.23 TB/s against 6.02-6.08
+    // with the small blocks, which are faster above).
     k_quad<T, true, kTB, 1><<<grid2(items / kTB), kTB, 0, g_stream>>>((V*)h->d, items, sorted_ins<2>(ins),
                                                                            uint64_t(1) << pl, uint64_t(1) << ph, M);
+  else if (items >= kTiledMin)
+    k_quad<T, true, TB, TP><<<grid2(items / (TB * TP)), TB, 0, g_stream>>>((V*)h->d, items, sorted_ins<2>(ins),
+                                                                            uint64_t(1) << pl, uint64_t(1) << ph, M);
   else
     k_quad<T, false, kSB, 2><<<grid2(grid_for(items, kSB * 2)), kSB, 0, g_stream>>>((V*)h->d, items, sorted_ins<2>(ins),
                                                                             uint64_t(1) << pl, uint64_t(1) << ph, M);

@@ -91,8 +91,9 @@ __device__ __forceinline__ uint64_t item_base(uint64_t k, const Ins<K>& ins)
 //    item space instead of a 1-in-8 interleave.
 //  * Split blocks: a block of B*P items takes SUB sub-ranges of B*P/SUB items spaced
 //    2^kSubLog items apart.  Keeping a block's partner streams a few tens of KiB apart is
-//    what lets the pair update run at ~6.3 TB/s at every target position (a single
-//    contiguous range per block stays near 5.6-5.9 TB/s for far-apart partners).
+//    what lets the pair update run at 6.3-6.8 TB/s at every target position (a single
+//    contiguous range per block stays near 5.6-5.9 TB/s for far-apart partners); the
+//    fastest shape for complex128 is 128 threads x 2 items in 4 runs of 64 items.
 // Both need a power-of-two item count >= kTiledMin; smaller states use the plain mapping.
 constexpr unsigned kSubLog = 11;
 constexpr int kSub = 4;

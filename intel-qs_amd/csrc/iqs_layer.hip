@@ -33,19 +33,39 @@ constexpr int kLB = 256;          // threads per workgroup
 constexpr int kLowQ = 6;          // positions handled by the lane-shuffle pass
 constexpr int kLowP = 4;          // amplitudes per lane in the lane-shuffle pass
 constexpr int kRegQ = 4;          // positions per register pass
-constexpr unsigned kMaxGrid = 2048;    // the partial buffer holds 2 * kMaxGrid sums
+constexpr unsigned kMaxGrid = 2048;    // workgroups of the grid-stride register passes (small states)
+// Lane passes: kLowRows rows of kLB * kLowP amplitudes per workgroup (28-qubit complex128:
+// forward 4.84 -> 5.45 TB/s, adjoint 4.07 -> 4.35 against 2048 workgroups with 128 rows each;
+// the QAOA form also builds its cut tables per workgroup, a few entries per thread).
+constexpr uint64_t kLowRows = 4;
+constexpr uint64_t kLowMaxGrid = uint64_t(1) << 22;
+// Register passes over >= kRegTileMin groups run one group per thread and one tile of kLB
+// groups per workgroup (XCD-contiguous, split into kRegSub runs of kLB / kRegSub groups
+// 2^kSubLog groups apart): 28-qubit complex128 forward pass 5.31 -> 5.86 TB/s against a
+// persistent grid-stride loop over 2048 workgroups.
+constexpr int kRegSub = 2;
+constexpr uint64_t kRegTileMin = uint64_t(1) << 16;
 
 struct LayerWork {
   double* part = nullptr;
+  size_t have = 0;
   std::vector<double> host;
 };
 LayerWork g_lw;
 std::mutex g_lm;
 
-int part_buf()
+// Partial-sum buffer of at least `count` doubles (2 * kMaxGrid at the least).
+int part_buf(size_t count = 0)
 {
-  if (!g_lw.part && hipMalloc((void**)&g_lw.part, sizeof(double) * 2 * kMaxGrid) != hipSuccess)
-    return iqsi::fail(IQSH_ERR_ALLOC, "layer reduction buffer");
+  count = std::max<size_t>(count, 2 * kMaxGrid);
+  if (g_lw.have >= count) return IQSH_OK;
+  IQSI_TRY(hipStreamSynchronize(iqsi::stream()));
+  if (g_lw.part) IQSI_TRY(hipFree(g_lw.part));
+  g_lw.part = nullptr;
+  g_lw.have = 0;
+  if (hipMalloc((void**)&g_lw.part, sizeof(double) * count) != hipSuccess)
+    return iqsi::fail(IQSH_ERR_ALLOC, "layer reduction buffer of %zu doubles", count);
+  g_lw.have = count;
   return IQSH_OK;
 }
 
@@ -64,7 +84,9 @@ __device__ __forceinline__ double im_term(const V& y, const Cx<double>& a, const
   return (double)y.x * gi - (double)y.y * gr;
 }
 
-__device__ __forceinline__ void wg_sum_store(double v, double* out)
+// Workgroup sum of v (fixed order: lanes by xor-shuffle, then the waves in order) stored
+// at out[slot] by thread 0.
+__device__ __forceinline__ void wg_sum_store(double v, double* out, uint64_t slot)
 {
   __shared__ double sh[kLB / 64];
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -74,8 +96,18 @@ __device__ __forceinline__ void wg_sum_store(double v, double* out)
   if (threadIdx.x == 0) {
     double s = 0;
     for (int w = 0; w < kLB / 64; ++w) s += sh[w];
-    out[blockIdx.x] = s;
+    out[slot] = s;
   }
+}
+__device__ __forceinline__ void wg_sum_store(double v, double* out) { wg_sum_store(v, out, blockIdx.x); }
+
+// out[0] = sum of part[0 .. n-1] in a fixed order (one workgroup): thread t adds part[t],
+// part[t + kLB], ... in sequence, then the workgroup sum.
+__global__ __launch_bounds__(kLB) void k_sum_part(const double* __restrict__ part, uint64_t n, double* __restrict__ out)
+{
+  double v = 0;
+  for (uint64_t i = threadIdx.x; i < n; i += kLB) v += part[i];
+  wg_sum_store(v, out, 0);
 }
 
 template <typename V>
@@ -138,18 +170,19 @@ __global__ __launch_bounds__(kLB) void k_layer_low(typename Vec<T>::t* __restric
 // before the rotations; adjoint (run as the last pass of the backward step, after every
 // other rotation of the layer is undone), the rotation terms, the undo, then the phase term
 // cut_i Im(conj(lam_i) psi_i) and psi, lam *= exp(+i gamma cut) (cs built with -gamma).
-// part receives two sums per workgroup (rotations, phase).
-template <typename T, bool ADJ>
+// part receives two sums per workgroup (rotations, phase).  W: weighted cuts (byte tables
+// of weight sums in LDS); unit weights only need the incidence tables and the phase table.
+template <typename T, bool ADJ, bool W>
 __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict__ psi, typename Vec<T>::t* __restrict__ lam,
                                                   uint64_t N, int L, Mat2<T> u, Mat2<double> g, CutSpec cs,
                                                   double* __restrict__ part)
 {
   using V = typename Vec<T>::t;
   __shared__ uint64_t tab[8][256];
-  __shared__ double wt[8][256];
+  __shared__ double wt[W ? 8 : 1][256];
   __shared__ Cx<T> ph[65];
   build_tables(tab, cs.inc);
-  if (cs.weighted) build_weight_tables(wt, cs);
+  if (W) build_weight_tables(wt, cs);
   for (unsigned c = threadIdx.x; c <= cs.ne && c < 65; c += kLB) ph[c] = Cx<T>{(T)cs.ph_re[c], (T)cs.ph_im[c]};
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -169,7 +202,7 @@ __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict
       if (ADJ) y[j] = ldg(&lam[r0 + (uint64_t)j * kLB]);
       int ic;
       cut[j] = cut_value(tab, wt, cs, r0 + (uint64_t)j * kLB, &ic);
-      if (!cs.weighted) {
+      if (!W) {
         f[j] = ph[ic];
       } else {
         double sn, co;
@@ -214,7 +247,10 @@ __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict
 
 // Register pass over M positions ins (all >= 6): each thread holds the 2^M amplitudes of a
 // group.  The generator terms of a qubit are taken before any of this pass's gates.
-template <typename T, bool ADJ, int M>
+// TILED: one group per thread, grid = items / kLB workgroups (items a power of two >=
+// kRegTileMin), one partial per workgroup at part[block_id()]; otherwise a grid-stride loop
+// over one contiguous chunk of groups per workgroup.
+template <typename T, bool ADJ, int M, bool TILED>
 __global__ __launch_bounds__(kLB) void k_layer_reg(typename Vec<T>::t* __restrict__ psi, typename Vec<T>::t* __restrict__ lam,
                                                    uint64_t items, Ins<M> ins, Mat2<T> u, Mat2<double> g,
                                                    double* __restrict__ part)
@@ -231,13 +267,22 @@ __global__ __launch_bounds__(kLB) void k_layer_reg(typename Vec<T>::t* __restric
     off[j] = o;
   }
   double acc = 0;
-  // One contiguous chunk of groups per workgroup, chunks XCD-contiguous (workgroups are
-  // dispatched round-robin over the 8 XCDs).
-  const uint64_t G = gridDim.x;
-  const uint64_t lb = (G & 7) ? blockIdx.x : xcd_block(blockIdx.x, G);
-  const uint64_t chunk = (items + G - 1) / G;
-  const uint64_t k0 = lb * chunk, k1 = (k0 + chunk < items) ? k0 + chunk : items;
-  for (uint64_t k = k0 + threadIdx.x; k < k1; k += kLB) {
+  // Persistent form: one contiguous chunk of groups per workgroup, chunks XCD-contiguous
+  // (workgroups are dispatched round-robin over the 8 XCDs).  Tiled form: one group.
+  uint64_t k0, k1, step;
+  if (TILED) {
+    k0 = tiled_item<kLB, 1, kRegSub, kSubLog>(xcd_block(block_id(), grid_blocks()), threadIdx.x);
+    k1 = k0 + 1;
+    step = 1;
+  } else {
+    const uint64_t G = gridDim.x;
+    const uint64_t lb = (G & 7) ? blockIdx.x : xcd_block(blockIdx.x, G);
+    const uint64_t chunk = (items + G - 1) / G;
+    k0 = lb * chunk + threadIdx.x;
+    k1 = (lb * chunk + chunk < items) ? lb * chunk + chunk : items;
+    step = kLB;
+  }
+  for (uint64_t k = k0; k < k1; k += step) {
     const uint64_t base = item_base(k, ins);
     V x[D], y[D];
 #pragma unroll
@@ -276,12 +321,27 @@ __global__ __launch_bounds__(kLB) void k_layer_reg(typename Vec<T>::t* __restric
       if (ADJ) stg(y[j], &lam[base | off[j]]);
     }
   }
-  if (ADJ) wg_sum_store(acc, part);
+  if (ADJ) wg_sum_store(acc, part, TILED ? block_id() : blockIdx.x);
 }
 
 unsigned grid_for_work(uint64_t work_items)
 {
   return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((work_items + kLB - 1) / kLB, kMaxGrid));
+}
+
+// Sums of the per-workgroup partials of one adjoint pass, in a fixed order on the device:
+// nsum groups of G partials stored one after the other; the sum of group i is added to
+// *out[i].  The partial buffer must hold nsum * (G + 1) doubles.
+int device_sums(uint64_t G, int nsum, double* const* out)
+{
+  for (int i = 0; i < nsum; ++i)
+    k_sum_part<<<1, kLB, 0, iqsi::stream()>>>(g_lw.part + i * G, G, g_lw.part + nsum * G + i);
+  IQSI_TRY(hipGetLastError());
+  double s[2] = {0, 0};
+  IQSI_TRY(hipMemcpyAsync(s, g_lw.part + nsum * G, sizeof(double) * nsum, hipMemcpyDeviceToHost, iqsi::stream()));
+  IQSI_TRY(hipStreamSynchronize(iqsi::stream()));
+  for (int i = 0; i < nsum; ++i) *out[i] += s[i];
+  return IQSH_OK;
 }
 
 // Sum the partials of one adjoint pass (in order) into *acc.
@@ -304,9 +364,23 @@ int reg_pass(iqsh_handle h, iqsh_handle lam, unsigned p0, const Mat2<T>& u, cons
   Ins<M> ins;
   for (int b = 0; b < M; ++b) ins.p[b] = p0 + b;
   const uint64_t items = h->size() >> M;
+  V* lp = ADJ ? (V*)lam->d : nullptr;
+  // The complex128 adjoint pass over 4 positions holds 32 amplitudes per thread and keeps the
+  // grid-stride form (28 qubits: 3.21 ms against 3.29 ms tiled).
+  const bool tiled = items >= kRegTileMin && !(ADJ && M == 4 && sizeof(T) == 8);
+  if (tiled) {
+    const uint64_t G = items / kLB;
+    if (ADJ)
+      if (int rc = part_buf(G + 1)) return rc;
+    const dim3 grid((unsigned)std::min<uint64_t>(G, 65536), (unsigned)((G + 65535) / 65536), 1);
+    k_layer_reg<T, ADJ, M, true><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, items, ins, u, g, g_lw.part);
+    IQSI_TRY(hipGetLastError());
+    if (!ADJ) return IQSH_OK;
+    double* out[1] = {acc};
+    return device_sums(G, 1, out);
+  }
   const unsigned grid = grid_for_work(items);
-  k_layer_reg<T, ADJ, M><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, ADJ ? (V*)lam->d : nullptr, items, ins, u, g,
-                                                             g_lw.part);
+  k_layer_reg<T, ADJ, M, false><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, items, ins, u, g, g_lw.part);
   if (!ADJ) {
     IQSI_TRY(hipGetLastError());
     return IQSH_OK;
@@ -328,29 +402,26 @@ int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm,
   // Forward: the lane pass first (it carries the phase layer, applied before the mixer);
   // adjoint: last (the phase is undone after the whole mixer).
   auto low_pass = [&]() -> int {
-    const uint64_t rows = h->size() / kLowP;
-    const unsigned grid = grid_for_work(rows);
-    if (cs)
-      k_qaoa_low<T, ADJ><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, ADJ ? (V*)lam->d : nullptr, h->size(), L, u, g,
-                                                            *cs, g_lw.part);
+    // The QAOA form builds its cut tables per workgroup: a persistent grid of <= kMaxGrid
+    // workgroups (28 qubits: 1.87 ms forward against 2.08 ms with kLowRows rows each).
+    const uint64_t rows = h->size() / (kLB * kLowP);
+    const unsigned grid = cs ? (unsigned)std::min<uint64_t>(rows, kMaxGrid)
+                             : (unsigned)std::min<uint64_t>(std::max<uint64_t>(1, rows / kLowRows), kLowMaxGrid);
+    if (ADJ)
+      if (int rc = part_buf(2 * (uint64_t)grid + 2)) return rc;
+    V* lp = ADJ ? (V*)lam->d : nullptr;
+    if (cs && cs->weighted)
+      k_qaoa_low<T, ADJ, true><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, h->size(), L, u, g, *cs, g_lw.part);
+    else if (cs)
+      k_qaoa_low<T, ADJ, false><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, h->size(), L, u, g, *cs, g_lw.part);
     else
       k_layer_low<T, ADJ><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, ADJ ? (V*)lam->d : nullptr, h->size(), L, u, g,
                                                              g_lw.part);
-    if (!ADJ) {
-      IQSI_TRY(hipGetLastError());
-      return IQSH_OK;
-    }
-    if (int rc = finish_pass(grid, &acc)) return rc;
-    if (cs) {
-      // The phase sums follow the rotation sums in the partial buffer.
-      double s = 0;
-      IQSI_TRY(hipMemcpyAsync(g_lw.host.data(), g_lw.part + grid, sizeof(double) * grid, hipMemcpyDeviceToHost,
-                              iqsi::stream()));
-      IQSI_TRY(hipStreamSynchronize(iqsi::stream()));
-      for (unsigned b = 0; b < grid; ++b) s += g_lw.host[b];
-      accg += s;
-    }
-    return IQSH_OK;
+    IQSI_TRY(hipGetLastError());
+    if (!ADJ) return IQSH_OK;
+    // The phase sums (QAOA form) follow the rotation sums in the partial buffer.
+    double* out[2] = {&acc, &accg};
+    return device_sums(grid, cs ? 2 : 1, out);
   };
   if (!ADJ)
     if (int rc = low_pass()) return rc;

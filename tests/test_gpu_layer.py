@@ -1,5 +1,6 @@
 """Layers of one gate on every qubit (iqsh_apply_1q_all / iqsh_adjoint_1q_all): the forward
-layer against the oracle applying the gate qubit by qubit; the adjoint layer against the
+layer against the oracle applying the gate qubit by qubit (20 qubits and up: register
+passes of one tile per workgroup; below: the grid-stride form); the adjoint layer against the
 per-qubit adjoint step on the GPU and against sum_q 2 Im <lam| G_q |psi> computed with the
 oracle's 1q kernel on the host (G_q psi for each q), then the oracle's per-qubit undo."""
 import numpy as np
@@ -35,7 +36,7 @@ def _gpu(h, dt):
 
 
 @pytest.mark.parametrize("dt", ["c128", "c64"])
-@pytest.mark.parametrize("n", [10, 11, 13, 17])
+@pytest.mark.parametrize("n", [10, 11, 13, 17, 20])
 def test_layer_forward(dt, n):
     h = _state(n, dt, n)
     u = _unitary(100 + n)
@@ -47,7 +48,7 @@ def test_layer_forward(dt, n):
 
 
 @pytest.mark.parametrize("dt", ["c128", "c64"])
-@pytest.mark.parametrize("n", [10, 12, 15])
+@pytest.mark.parametrize("n", [10, 12, 15, 20])
 def test_layer_adjoint(dt, n):
     psi = _state(n, dt, 2 * n)
     lam = _state(n, dt, 2 * n + 1)
@@ -92,12 +93,12 @@ def _graph(n, seed, weighted):
 
 @pytest.mark.parametrize("dt", ["c128", "c64"])
 @pytest.mark.parametrize("weighted", [False, True])
-def test_qaoa_layer_fused_equals_unfused(dt, weighted):
+@pytest.mark.parametrize("n", [12, 20])
+def test_qaoa_layer_fused_equals_unfused(dt, weighted, n):
     """iqsh_qaoa_layer = iqsh_qaoa_phase then the mixer on every qubit; the fused adjoint =
     iqsh_adjoint_1q_all then iqsh_qaoa_phase_adjoint (gradients and states); the forward
     layer also against the oracle (cost diagonal from the edge list, 1q kernel per qubit)."""
     from iqs_amd import gates as Gt
-    n = 12
     edges = _graph(n, 3, weighted)
     gamma, beta = 0.83, 0.41
     psi = _state(n, dt, 41)

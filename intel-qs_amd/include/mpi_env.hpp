@@ -4,6 +4,7 @@
 /// communicators.  The whole register lives in one GPU's HBM; nothing here communicates.
 #pragma once
 
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 
@@ -47,6 +48,17 @@ class Environment
 inline void Barrier() {}
 inline void PoolBarrier() {}
 inline void StateBarrier() {}
+
+/// Print from state rank 0 (src/mpi_env.cpp:66-87, non-MPI build): the string alone, or
+/// with all = true prefixed by "[|state id>:rank]".
+inline void StatePrint(std::string s, bool all = false)
+{
+  if (all) std::printf("[|%d>:%3d] %s\n", Environment::GetStateId(), Environment::GetStateRank(), s.c_str());
+  else std::printf("%s\n", s.c_str());
+  std::fflush(stdout);
+}
+inline void PoolPrint(std::string s, bool all = false) { StatePrint(s, all); }
+inline void Print(std::string s, bool all = false) { StatePrint(s, all); }
 
 }  // namespace mpi
 }  // namespace iqs

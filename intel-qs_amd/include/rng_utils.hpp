@@ -10,6 +10,7 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "mpi_env.hpp"
@@ -24,6 +25,11 @@ class RandomNumberGenerator
   explicit RandomNumberGenerator(RandomNumberGenerator* src) { *this = *src; }
 
   std::size_t GetSeed() const { return seed_; }
+  /// Numbers drawn or skipped per stream since the last seeding (a Gaussian counts twice),
+  /// as include/rng_utils.hpp:82-89 of the reference.
+  std::size_t GetNumGeneratedOrSkippedLocalNumbers() const { return n_local_; }
+  std::size_t GetNumGeneratedOrSkippedStateNumbers() const { return n_state_; }
+  std::size_t GetNumGeneratedOrSkippedPoolNumbers() const { return n_pool_; }
   void SetSeedStreamPtrs(std::size_t seed)
   {
     SetSeedStreamPtrs(seed, mpi::Environment::GetStateId(), mpi::Environment::GetNumStates(),
@@ -82,5 +88,18 @@ class RandomNumberGenerator
   std::uniform_real_distribution<Type> u_{Type(0), Type(1)};
   std::normal_distribution<Type> n_{Type(0), Type(1)};
 };
+
+/// Fisher-Yates shuffle in the reference's draw order (src/rng_utils.cpp:403-423): for
+/// ra = size-1 down to 1, one integer rb in [0, ra + 1) from `shared`, then swap a[ra], a[rb].
+template <typename Type, typename TypeFloat>
+void ShuffleFisherYates(std::vector<Type>& array, RandomNumberGenerator<TypeFloat>* rng, std::string shared = "local")
+{
+  if (!rng) throw std::invalid_argument("ShuffleFisherYates: null generator");
+  for (int ra = (int)array.size() - 1; ra > 0; --ra) {
+    int rb;
+    rng->RandomIntegersInRange(&rb, 1, 0, ra + 1, shared);
+    if (ra != rb) std::swap(array[ra], array[rb]);
+  }
+}
 
 }  // namespace iqs

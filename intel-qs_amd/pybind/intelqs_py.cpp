@@ -107,6 +107,10 @@ void bind_rng(py::module_& m, const char* name)
       .def("GetSeed", &R::GetSeed)
       .def("SetSeedStreamPtrs", py::overload_cast<std::size_t>(&R::SetSeedStreamPtrs))
       .def("SkipeAhead", &R::SkipAhead)
+      // extension: the stream counters of the C++ class (include/rng_utils.hpp:82-89)
+      .def("GetNumGeneratedOrSkippedLocalNumbers", &R::GetNumGeneratedOrSkippedLocalNumbers)
+      .def("GetNumGeneratedOrSkippedStateNumbers", &R::GetNumGeneratedOrSkippedStateNumbers)
+      .def("GetNumGeneratedOrSkippedPoolNumbers", &R::GetNumGeneratedOrSkippedPoolNumbers)
       .def("GetUniformRandomNumbers",
            [](R& r, std::size_t size, B a, B b, std::string shared) {
              std::vector<B> v(size);
@@ -313,6 +317,14 @@ PYBIND11_MODULE(intelqs_py, m)
   m.def("EnvFinalizeDummyRanks", []() {});
 
   bind_rng<double>(m, "RandomNumberGenerator");
+  // extension: the C++ utility ShuffleFisherYates (src/rng_utils.cpp:403-423) on a list of ints
+  m.def(
+      "ShuffleFisherYates",
+      [](std::vector<int> a, iqs::RandomNumberGenerator<double>& rng, std::string shared) {
+        iqs::ShuffleFisherYates(a, &rng, shared);
+        return a;
+      },
+      py::arg("array"), py::arg("rng"), py::arg("shared") = "local");
   bind_rng<float>(m, "RandomNumberGeneratorSP");  // extension: the generator QubitRegisterSP takes
 
   bind_chi<4>(m, "CM4x4", "<ChiMatrix for 1-qubit channel>");

@@ -99,3 +99,43 @@ def test_rng_streams_and_integers_match_restatement():
     r2.SetSeedStreamPtrs(777)
     r2.UniformRandomNumbers(buf, 4, 0.0, 1.0, "pool")
     assert list(buf) == _mt_uniform(777, 4)
+
+
+def test_rng_counters_skip_and_shuffle():
+    """Stream counters as the reference's unit test checks them
+    (unit_test/include/random_number_generator_test.hpp:48-62, 154-161): n uniforms count n
+    on their stream only, a Gaussian counts twice, SkipAhead(k) draws and counts k uniforms
+    (src/rng_utils.cpp:101-112); ShuffleFisherYates (src/rng_utils.cpp:403-423) restated with
+    integers floor(u (ra + 1)) drawn from the local stream std::mt19937(seed + 1 + num_states +
+    pool rank) = std::mt19937(seed + 2), ra = n-1 .. 1."""
+    import numpy as np
+    import intelqs_py as m
+    r = m.RandomNumberGenerator()
+    r.SetSeedStreamPtrs(4242)
+    r.GetUniformRandomNumbers(37, 0.0, 1.0, "local")
+    assert (r.GetNumGeneratedOrSkippedLocalNumbers(), r.GetNumGeneratedOrSkippedStateNumbers(),
+            r.GetNumGeneratedOrSkippedPoolNumbers()) == (37, 0, 0)
+    r.GaussianRandomNumbers(np.zeros(5), 5, "pool")
+    r.SkipeAhead(11, "state")
+    assert (r.GetNumGeneratedOrSkippedLocalNumbers(), r.GetNumGeneratedOrSkippedStateNumbers(),
+            r.GetNumGeneratedOrSkippedPoolNumbers()) == (37, 11, 10)
+    # skipping k numbers = drawing k numbers
+    a, b = m.RandomNumberGenerator(), m.RandomNumberGenerator()
+    a.SetSeedStreamPtrs(9)
+    b.SetSeedStreamPtrs(9)
+    a.SkipeAhead(7, "local")
+    assert a.GetUniformRandomNumbers(3, 0.0, 1.0, "local") == b.GetUniformRandomNumbers(10, 0.0, 1.0, "local")[7:]
+    r.SetSeedStreamPtrs(4242)
+    assert r.GetNumGeneratedOrSkippedLocalNumbers() == 0
+    # shuffle: same draws, same swaps
+    n = 12
+    s = m.RandomNumberGenerator()
+    s.SetSeedStreamPtrs(31)
+    got = m.ShuffleFisherYates(list(range(n)), s, "local")
+    u = _mt_uniform(33, n - 1)
+    want = list(range(n))
+    for i, ra in enumerate(range(n - 1, 0, -1)):
+        rb = int(np.floor(0 + u[i] * (ra + 1)))
+        want[ra], want[rb] = want[rb], want[ra]
+    assert got == want and sorted(got) == list(range(n))
+    assert s.GetNumGeneratedOrSkippedLocalNumbers() == n - 1

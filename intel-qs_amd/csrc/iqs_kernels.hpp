@@ -626,6 +626,99 @@ __global__ __launch_bounds__(kBlock) void k_block5_mfma(typename Vec<T>::t* __re
   }
 }
 
+// The same product with A in LDS instead of registers: the 64 x 64 real matrix is stored
+// in fragment order, sA[(16 mb + ks) * 64 + lane] = A[16 mb + lane%16][4 ks + lane/16], so
+// every MFMA operand read is one conflict-free 512-byte LDS access, and the 128 VGPRs (f64)
+// that held A are free for more resident waves.  PF: load the next tile while this one is
+// multiplied.  The MFMA sequence and operands are those of k_block5_mfma (same results).
+template <typename T, int TPW, bool NT, bool PF>
+__global__ __launch_bounds__(kBlock) void k_block5_mfma_lds(typename Vec<T>::t* __restrict__ st, uint64_t ntiles,
+                                                            Ins<5> ins, const typename Vec<T>::t* __restrict__ U)
+{
+  using V = typename Vec<T>::t;
+  using Acc = typename MfmaAcc<T>::t;
+  __shared__ T sA[64 * 64];
+  for (int e = threadIdx.x; e < 64 * 64; e += kBlock) {
+    const int mb = e >> 10, ks = (e >> 6) & 15, l = e & 63;
+    const int R = 16 * mb + (l & 15), Cc = 4 * ks + (l >> 4);
+    const V u = U[(R & 31) * 32 + (Cc & 31)];
+    sA[e] = (R < 32) ? ((Cc < 32) ? u.x : -u.y) : ((Cc < 32) ? u.y : u.x);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, c = lane >> 4, gl = lane & 15;
+  const uint64_t G = grid_blocks();
+  const uint64_t bb = (G & 7) ? block_id() : xcd_block(block_id(), G);
+  const uint64_t wave = bb * (kBlock / 64) + (threadIdx.x >> 6);
+  const uint64_t t0 = wave * TPW;
+  if (t0 >= ntiles) return;
+  uint64_t offin[8], offout[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int rin = c + 4 * j;
+    const int i = j & 3;
+    const int rin4 = (sizeof(T) == 8) ? (c + 4 * i) : (4 * c + i);
+    const int rout = (j < 4) ? rin4 : (16 + rin4);
+    uint64_t oi = 0, oo = 0;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+      if ((rin >> b) & 1) oi |= uint64_t(1) << ins.p[b];
+      if ((rout >> b) & 1) oo |= uint64_t(1) << ins.p[b];
+    }
+    offin[j] = oi;
+    offout[j] = oo;
+  }
+  const uint64_t t1 = (t0 + TPW < ntiles) ? t0 + TPW : ntiles;
+  uint64_t base = item_base(t0 * 16 + gl, ins);
+  V x[8];
+  if (PF) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = ldx<NT>(&st[base | offin[j]]);
+  }
+  for (uint64_t t = t0; t < t1; ++t) {
+    // Keep the A reads inside the loop: hoisted out of it, A would occupy registers again.
+    __asm__ volatile("" ::: "memory");
+    const bool more = PF && t + 1 < t1;
+    const uint64_t nbase = more ? item_base((t + 1) * 16 + gl, ins) : base;
+    V xn[PF ? 8 : 1];
+    if (PF) {
+      if (more) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) xn[j] = ldx<NT>(&st[nbase | offin[j]]);
+      }
+    } else {
+      base = item_base(t * 16 + gl, ins);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) x[j] = ldx<NT>(&st[base | offin[j]]);
+    }
+    Acc d[4];
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb) d[mb] = Acc{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) {
+      const T bv = (ks < 8) ? x[ks].x : x[ks - 8].y;
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) d[mb] = mfma16x16x4(sA[(16 * mb + ks) * 64 + lane], bv, d[mb]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      V y0, y1;
+      y0.x = d[0][i];
+      y0.y = d[2][i];
+      y1.x = d[1][i];
+      y1.y = d[3][i];
+      stx<NT>(y0, &st[base | offout[i]]);
+      stx<NT>(y1, &st[base | offout[4 + i]]);
+    }
+    if (PF) {
+      if (more) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) x[j] = xn[PF ? j : 0];
+      }
+      base = nbase;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // Fills.
 template <typename T>

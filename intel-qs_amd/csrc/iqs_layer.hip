@@ -178,6 +178,9 @@ __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict
                                                   double* __restrict__ part)
 {
   using V = typename Vec<T>::t;
+  // The adjoint holds psi and lambda: 2 amplitudes of each per lane keep it at 4 waves per
+  // SIMD instead of 2.
+  constexpr int P = ADJ ? 2 : kLowP;
   __shared__ uint64_t tab[8][256];
   __shared__ double wt[W ? 8 : 1][256];
   __shared__ Cx<T> ph[65];
@@ -187,17 +190,17 @@ __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict
   __syncthreads();
   const int lane = threadIdx.x & 63;
   double acc = 0, accg = 0;
-  const uint64_t rows = N / (kLB * kLowP), G = gridDim.x;
+  const uint64_t rows = N / (kLB * P), G = gridDim.x;
   const uint64_t lb = (G & 7) ? blockIdx.x : xcd_block(blockIdx.x, G);
   const uint64_t chunk = (rows + G - 1) / G;
   const uint64_t b0 = lb * chunk, b1 = (b0 + chunk < rows) ? b0 + chunk : rows;
   for (uint64_t rb = b0; rb < b1; ++rb) {
-    const uint64_t r0 = rb * (kLB * kLowP) + threadIdx.x;
-    V x[kLowP], y[kLowP];
-    Cx<T> f[kLowP];
-    double cut[kLowP];
+    const uint64_t r0 = rb * (kLB * P) + threadIdx.x;
+    V x[P], y[P];
+    Cx<T> f[P];
+    double cut[P];
 #pragma unroll
-    for (int j = 0; j < kLowP; ++j) {
+    for (int j = 0; j < P; ++j) {
       x[j] = ldg(&psi[r0 + (uint64_t)j * kLB]);
       if (ADJ) y[j] = ldg(&lam[r0 + (uint64_t)j * kLB]);
       int ic;
@@ -215,7 +218,7 @@ __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict
       const int msk = 1 << q;
       const bool hi = (lane >> q) & 1;
 #pragma unroll
-      for (int j = 0; j < kLowP; ++j) {
+      for (int j = 0; j < P; ++j) {
         const V xp = shfl_v(x[j], msk);
         const V x0 = hi ? xp : x[j], x1 = hi ? x[j] : xp;
         if (ADJ) {
@@ -228,7 +231,7 @@ __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict
       }
     }
 #pragma unroll
-    for (int j = 0; j < kLowP; ++j) {
+    for (int j = 0; j < P; ++j) {
       if (ADJ) {
         accg += cut[j] * ((double)y[j].x * x[j].y - (double)y[j].y * x[j].x);
         x[j] = cmul(f[j], x[j]);

@@ -161,6 +161,13 @@ constexpr int kTB = 1024, kTP = 2;  // tiled scale / diag kernels; pair and quad
 template <typename T> struct PairTile { static constexpr int B = 1024, P = 2, QP = 1; };
 template <> struct PairTile<double> { static constexpr int B = 128, P = 2, QP = 2; };
 constexpr int kLB = 256, kLP = 2;   // lane-shuffle 1q kernel
+// Lane-shuffle 1q kernel and 2q diagonal, per precision (30q complex128 sweep): lane kernel
+// 64 x 2 (one wavefront per workgroup) 6.36 -> 6.60 TB/s against 256 x 2; diagonal 128 x 2
+// 6.61 -> 6.77 TB/s against 1024 x 2.  complex64 keeps the earlier shapes.
+template <typename T> struct LaneTile { static constexpr int B = kLB, P = kLP; };
+template <> struct LaneTile<double> { static constexpr int B = 64, P = 2; };
+template <typename T> struct DiagTile { static constexpr int B = kTB, P = kTP; };
+template <> struct DiagTile<double> { static constexpr int B = 128, P = 2; };
 
 // Index bits inside one 128-byte cache line: 3 for complex128, 4 for complex64.
 template <typename T>
@@ -204,7 +211,8 @@ int launch_lane(iqsh_handle h, unsigned t, const Ins<KC>& cins, uint64_t ctrl, c
 {
   using V = typename Vec<T>::t;
   const uint64_t items = h->size() >> KC;
-  k_pair_lane<T, KC, COND, MOVE, kLB, kLP><<<grid2(items / (kLB * kLP)), kLB, 0, g_stream>>>((V*)h->d, t, cins, ctrl,
+  constexpr int LB = LaneTile<T>::B, LP = LaneTile<T>::P;
+  k_pair_lane<T, KC, COND, MOVE, LB, LP><<<grid2(items / (LB * LP)), LB, 0, g_stream>>>((V*)h->d, t, cins, ctrl,
                                                                                                   mat2<T>(m));
   CHECK_LAUNCH();
   return IQSH_OK;
@@ -272,7 +280,8 @@ int launch_diag2(iqsh_handle h, unsigned p1, unsigned p2, Cx<T> d0, Cx<T> d1, Cx
   using V = typename Vec<T>::t;
   const uint64_t N = h->size();
   if (N >= kTiledMin)
-    k_diag2<T, true, kTB, kTP><<<grid2(N / (kTB * kTP)), kTB, 0, g_stream>>>((V*)h->d, N, p1, p2, d0, d1, d2, d3);
+    k_diag2<T, true, DiagTile<T>::B, DiagTile<T>::P><<<grid2(N / (DiagTile<T>::B * DiagTile<T>::P)), DiagTile<T>::B, 0,
+                                                       g_stream>>>((V*)h->d, N, p1, p2, d0, d1, d2, d3);
   else
     k_diag2<T, false, kSB, kSP><<<grid2(grid_for(N, kSB * kSP)), kSB, 0, g_stream>>>((V*)h->d, N, p1, p2, d0, d1, d2, d3);
   CHECK_LAUNCH();

@@ -232,11 +232,13 @@ int launch_pair_cond(iqsh_handle h, unsigned t, uint64_t ctrl, const double* m, 
   Ins<1> ins;
   ins.p[0] = t;
   const uint64_t items = h->size() >> 1;
-  const unsigned grid = (unsigned)(items / (kTB * kTP));
+  // complex128: the pair kernel's 128 x 2 shape (30q: 6.0 -> 6.5 TB/s of full-pass traffic).
+  constexpr int TB = PairTile<T>::B, TP = PairTile<T>::P;
+  const uint64_t grid = items / (TB * TP);
   if (move)
-    k_pair_cond<T, true, kTB, kTP><<<grid2(grid), kTB, 0, g_stream>>>((V*)h->d, ins, ctrl, mat2<T>(m));
+    k_pair_cond<T, true, TB, TP><<<grid2(grid), TB, 0, g_stream>>>((V*)h->d, ins, ctrl, mat2<T>(m));
   else
-    k_pair_cond<T, false, kTB, kTP><<<grid2(grid), kTB, 0, g_stream>>>((V*)h->d, ins, ctrl, mat2<T>(m));
+    k_pair_cond<T, false, TB, TP><<<grid2(grid), TB, 0, g_stream>>>((V*)h->d, ins, ctrl, mat2<T>(m));
   CHECK_LAUNCH();
   return IQSH_OK;
 }
@@ -1069,6 +1071,19 @@ int launch_quad_lane(iqsh_handle h, unsigned pa, unsigned pb, const Mat4<T>& M, 
   using V = typename Vec<T>::t;
   const unsigned hi = std::max(pa, pb);
   const uint64_t items = hi >= 6 ? h->size() >> 1 : h->size();
+  // complex128 with the higher position >= 6: one-wavefront workgroups as the 1q lane kernel
+  // (30q: (0,29) 5.83 -> 6.08 TB/s, swap (1,28) 2.74 -> 3.02 algorithmic); both positions
+  // below 6 keep 256 x 2 workgroups (5.45 TB/s against 5.15).
+  constexpr int LB = LaneTile<T>::B, LP = LaneTile<T>::P;
+  if (hi >= 6 && LB != kLB) {
+    const uint64_t grid = items / (LB * LP);
+    if (move)
+      k_quad_lane<T, true, LB, LP><<<grid2(grid), LB, 0, g_stream>>>((V*)h->d, pa, pb, M);
+    else
+      k_quad_lane<T, false, LB, LP><<<grid2(grid), LB, 0, g_stream>>>((V*)h->d, pa, pb, M);
+    CHECK_LAUNCH();
+    return IQSH_OK;
+  }
   const uint64_t grid = items / (kLB * kLP);
   if (move)
     k_quad_lane<T, true, kLB, kLP><<<grid2(grid), kLB, 0, g_stream>>>((V*)h->d, pa, pb, M);

@@ -161,13 +161,17 @@ constexpr int kTB = 1024, kTP = 2;  // tiled scale / diag kernels; pair and quad
 template <typename T> struct PairTile { static constexpr int B = 1024, P = 2, QP = 1; };
 template <> struct PairTile<double> { static constexpr int B = 128, P = 2, QP = 2; };
 constexpr int kLB = 256, kLP = 2;   // lane-shuffle 1q kernel
-// Lane-shuffle 1q kernel and 2q diagonal, per precision (30q complex128 sweep): lane kernel
-// 64 x 2 (one wavefront per workgroup) 6.36 -> 6.60 TB/s against 256 x 2; diagonal 128 x 2
-// 6.61 -> 6.77 TB/s against 1024 x 2.  complex64 keeps the earlier shapes.
+// Lane-shuffle 1q kernel and 2q diagonal, per precision (30q sweeps): complex128 lane kernel
+// 64 x 2 (one wavefront per workgroup) 6.36 -> 6.60 TB/s against 256 x 2, complex64 keeps
+// 256 x 2 (6.71); diagonal 128 x 2 for both: 6.61 -> 6.77 (complex128) against 1024 x 2.
 template <typename T> struct LaneTile { static constexpr int B = kLB, P = kLP; };
 template <> struct LaneTile<double> { static constexpr int B = 64, P = 2; };
-template <typename T> struct DiagTile { static constexpr int B = kTB, P = kTP; };
-template <> struct DiagTile<double> { static constexpr int B = 128, P = 2; };
+template <typename T> struct DiagTile { static constexpr int B = 128, P = 2; };  // complex64: 6.22 -> 6.54
+// Lane-shuffle 4x4, for a higher position >= 6 and below it: complex64 64 x 4 everywhere
+// (30q: (1,0) 4.98 -> 5.63 TB/s, (29,0) 5.86 -> 6.14); complex128 64 x 2 above ((0,29)
+// 5.83 -> 6.08, swap (1,28) 2.74 -> 3.02 algorithmic), 256 x 2 below (5.45 against 5.15).
+template <typename T> struct QuadLaneTile { static constexpr int B = 64, P = 4, BL = 64, PL = 4; };
+template <> struct QuadLaneTile<double> { static constexpr int B = 64, P = 2, BL = kLB, PL = kLP; };
 
 // Index bits inside one 128-byte cache line: 3 for complex128, 4 for complex64.
 template <typename T>
@@ -1071,24 +1075,21 @@ int launch_quad_lane(iqsh_handle h, unsigned pa, unsigned pb, const Mat4<T>& M, 
   using V = typename Vec<T>::t;
   const unsigned hi = std::max(pa, pb);
   const uint64_t items = hi >= 6 ? h->size() >> 1 : h->size();
-  // complex128 with the higher position >= 6: one-wavefront workgroups as the 1q lane kernel
-  // (30q: (0,29) 5.83 -> 6.08 TB/s, swap (1,28) 2.74 -> 3.02 algorithmic); both positions
-  // below 6 keep 256 x 2 workgroups (5.45 TB/s against 5.15).
-  constexpr int LB = LaneTile<T>::B, LP = LaneTile<T>::P;
-  if (hi >= 6 && LB != kLB) {
-    const uint64_t grid = items / (LB * LP);
+  using Q = QuadLaneTile<T>;
+  if (hi >= 6) {
+    const uint64_t grid = items / (Q::B * Q::P);
     if (move)
-      k_quad_lane<T, true, LB, LP><<<grid2(grid), LB, 0, g_stream>>>((V*)h->d, pa, pb, M);
+      k_quad_lane<T, true, Q::B, Q::P><<<grid2(grid), Q::B, 0, g_stream>>>((V*)h->d, pa, pb, M);
     else
-      k_quad_lane<T, false, LB, LP><<<grid2(grid), LB, 0, g_stream>>>((V*)h->d, pa, pb, M);
+      k_quad_lane<T, false, Q::B, Q::P><<<grid2(grid), Q::B, 0, g_stream>>>((V*)h->d, pa, pb, M);
     CHECK_LAUNCH();
     return IQSH_OK;
   }
-  const uint64_t grid = items / (kLB * kLP);
+  const uint64_t grid = items / (Q::BL * Q::PL);
   if (move)
-    k_quad_lane<T, true, kLB, kLP><<<grid2(grid), kLB, 0, g_stream>>>((V*)h->d, pa, pb, M);
+    k_quad_lane<T, true, Q::BL, Q::PL><<<grid2(grid), Q::BL, 0, g_stream>>>((V*)h->d, pa, pb, M);
   else
-    k_quad_lane<T, false, kLB, kLP><<<grid2(grid), kLB, 0, g_stream>>>((V*)h->d, pa, pb, M);
+    k_quad_lane<T, false, Q::BL, Q::PL><<<grid2(grid), Q::BL, 0, g_stream>>>((V*)h->d, pa, pb, M);
   CHECK_LAUNCH();
   return IQSH_OK;
 }

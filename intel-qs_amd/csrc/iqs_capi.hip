@@ -152,7 +152,7 @@ bool is_c(const double* a, double re, double im) { return a[0] == re && a[1] == 
 // Launchers
 
 // Launch configurations (see the tiling notes in iqs_kernels.hpp).
-constexpr int kTB = 1024, kTP = 2;  // tiled scale / diag kernels; pair and quad kernels for complex64
+constexpr int kTB = 1024, kTP = 2;  // pair and 4x4 kernels for complex64 (tiled)
 // Pair and 4x4 kernels, per precision (30q sweep of block size x items per thread x split
 // count): complex128 streams best with 128-thread blocks of 2 items split 4 ways (64-item
 // runs, one wavefront access each): dense 1q 6.15 -> 6.53 TB/s for targets >= 11, controlled
@@ -166,7 +166,9 @@ constexpr int kLB = 256, kLP = 2;   // lane-shuffle 1q kernel
 // 256 x 2 (6.71); diagonal 128 x 2 for both: 6.61 -> 6.77 (complex128) against 1024 x 2.
 template <typename T> struct LaneTile { static constexpr int B = kLB, P = kLP; };
 template <> struct LaneTile<double> { static constexpr int B = 64, P = 2; };
-template <typename T> struct DiagTile { static constexpr int B = 128, P = 2; };  // complex64: 6.22 -> 6.54
+// Diagonal and partial-state scale kernels: 128 x 2 for both precisions (2q diagonal,
+// complex64 6.22 -> 6.54 TB/s; Z / CZ scale 6.31 -> 6.48 complex128, 6.43 -> 6.60 complex64).
+template <typename T> struct DiagTile { static constexpr int B = 128, P = 2; };
 // Lane-shuffle 4x4, for a higher position >= 6 and below it: complex64 64 x 4 everywhere
 // (30q: (1,0) 4.98 -> 5.63 TB/s, (29,0) 5.86 -> 6.14); complex128 64 x 2 above ((0,29)
 // 5.83 -> 6.08, swap (1,28) 2.74 -> 3.02 algorithmic), 256 x 2 below (5.45 against 5.15).
@@ -272,8 +274,9 @@ int launch_scale(iqsh_handle h, const Ins<K>& ins, uint64_t mask, const double* 
     return launch_diag2<T>(h, pb, pa, d[0], d[1], d[2], d[3]);
   }
   if (items >= kTiledMin)
-    k_scale<T, K, true, kTB, kTP><<<grid2(items / (kTB * kTP)), kTB, 0, g_stream>>>((V*)h->d, items, ins, mask,
-                                                                                          cx<T>(f));
+    k_scale<T, K, true, DiagTile<T>::B, DiagTile<T>::P><<<grid2(items / (DiagTile<T>::B * DiagTile<T>::P)),
+                                                          DiagTile<T>::B, 0, g_stream>>>((V*)h->d, items, ins, mask,
+                                                                                         cx<T>(f));
   else
     k_scale<T, K, false, kSB, kSP><<<grid2(grid_for(items, kSB * kSP)), kSB, 0, g_stream>>>((V*)h->d, items, ins, mask, cx<T>(f));
   CHECK_LAUNCH();

@@ -115,7 +115,7 @@ def cpu_baseline(n: int, layer_ops, budget_s: float = 30.0):
                       f"(intel-qs src/, -O3 OpenMP), per-gate steady_clock"}
 
 
-def config1_against_reference(torch):
+def brick20_against_reference(torch):
     """The reference's CPU-runnable case: a 20-qubit ComplexDP brick circuit of depth 20
     (seed 20240601) with <Z_i> read-outs, run through the drop-in on the GPU and through the
     compiled reference (oracle/_ref/ref_driver, OpenMP) on this host.  Reports both wall
@@ -576,7 +576,7 @@ def main() -> int:
     log("HEA batch done")
     cpu = None if args.no_cpu else cpu_baseline(n, layers[0])
     if cpu is not None:
-        cpu["config1_20q"] = config1_against_reference(torch)
+        cpu["brick_20q_depth20"] = brick20_against_reference(torch)
         cpu["qaoa_16q_param_shift"] = None if args.no_qaoa else qaoa_param_shift_on_reference(torch)
         cpu["hea_batch_sample"] = None if args.no_batch else hea_sample_on_reference(torch)
     log("CPU baseline done")

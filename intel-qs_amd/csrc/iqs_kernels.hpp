@@ -801,6 +801,56 @@ __global__ __launch_bounds__(kBlock) void k_permute(typename Vec<T>::t* __restri
     stg(src[eval_tables(tab, bt.nbytes, j)], &dst[j]);
 }
 
+// Tiled bit permutation, for permutations that move low positions up (where the gather
+// above reads with a stride).  A tile is the set of 2^m destination indices that differ only
+// in the tile bits: the kPermLow lowest destination bits plus every destination bit whose
+// source position is below kPermLow (m <= 2 kPermLow).  The tile is read in source order
+// (tile bits sorted by source position, so consecutive threads read consecutive source
+// amplitudes), kept in LDS in destination order, and written in destination order.
+constexpr unsigned kPermLow = 5;
+struct PermTile {
+  unsigned m;          // tile bits
+  unsigned tb[16];     // destination bits of the tile, ascending
+  unsigned sd[16];     // i-th tile bit in source order: its index in tb
+  uint64_t ss[16];     // i-th tile bit in source order: 1 << its source position
+  unsigned nout;       // destination bits outside the tile
+  unsigned ob[64];     // those bits, ascending (bit i of the tile index sets ob[i])
+  uint64_t os[64];     // 1 << their source positions
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_permute_tiled(typename Vec<T>::t* __restrict__ dst,
+                                                          const typename Vec<T>::t* __restrict__ src, PermTile pt)
+{
+  using V = typename Vec<T>::t;
+  __shared__ V tile[1u << (2 * kPermLow)];
+  const uint64_t t = block_id();
+  uint64_t j0 = 0, s0 = 0;
+  for (unsigned i = 0; i < pt.nout; ++i)
+    if ((t >> i) & 1) {
+      j0 |= uint64_t(1) << pt.ob[i];
+      s0 |= pt.os[i];
+    }
+  const unsigned M = 1u << pt.m;
+  for (unsigned r = threadIdx.x; r < M; r += kBlock) {
+    uint64_t sa = s0;
+    unsigned d = 0;
+    for (unsigned i = 0; i < pt.m; ++i)
+      if ((r >> i) & 1) {
+        sa |= pt.ss[i];
+        d |= 1u << pt.sd[i];
+      }
+    tile[d] = ldg(&src[sa]);
+  }
+  __syncthreads();
+  for (unsigned r = threadIdx.x; r < M; r += kBlock) {
+    uint64_t ja = j0;
+    for (unsigned i = 0; i < pt.m; ++i)
+      if ((r >> i) & 1) ja |= uint64_t(1) << pt.tb[i];
+    stg(tile[r], &dst[ja]);
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // QAOA MaxCut: edge parity vector E(x) (bit e = [x_u != x_v]) from incidence tables; the cut
 // is popcount(E) (unit weights) or sum of weights over set bits of E (byte tables).

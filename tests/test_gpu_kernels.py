@@ -172,11 +172,22 @@ def test_kq_block_matches_kronecker(dt, positions):
     assert np.abs(sv.download() - want).max() < (1e-12 if dt == "c128" else 2e-5)
 
 
-@pytest.mark.parametrize("perm_seed", [1, 2, 3])
-def test_permute_bit_exact(dt, perm_seed):
-    n = 11
-    r = np.random.default_rng(perm_seed)
-    src = list(r.permutation(n))
+def _perm_cases():
+    """(n, src_pos): random permutations below and above 16 qubits (from 16 up, a
+    permutation that moves low positions up runs as a tiled transpose through LDS), the
+    reversal, a rotation, and an exchange of high positions only (gather kernel)."""
+    cases = [(11, 1), (11, 2), (11, 3), (17, 4), (20, 5)]
+    out = [(n, [int(x) for x in np.random.default_rng(s).permutation(n)]) for n, s in cases]
+    out.append((18, list(range(17, -1, -1))))
+    out.append((18, [(b + 1) % 18 for b in range(18)]))
+    out.append((18, [12 if b == 16 else 16 if b == 12 else b for b in range(18)]))
+    return out
+
+
+@pytest.mark.parametrize("case", _perm_cases(), ids=lambda c: f"n{c[0]}-{''.join(chr(97 + x) for x in c[1][:6])}")
+def test_permute_bit_exact(dt, case):
+    n, src = case
+    perm_seed = n
     h = rand_state(n, dt, perm_seed)
     sv = on_gpu(h, dt)
     sv.permute(src)

@@ -275,6 +275,14 @@ int iqsh_hea_run(iqsh_hea_handle h, int flags);
  * skips one.  The host form synchronizes; the device form copies on the stream. */
 int iqsh_hea_results(iqsh_hea_handle h, double *cost, double *grad);
 int iqsh_hea_results_device(iqsh_hea_handle h, double *cost_dev, double *grad_dev);
+/* Both forms first read the run's control words (the device form therefore synchronizes
+ * too) and return IQSH_ERR_HIP, copying nothing, when a dependency wait gave up or a task
+ * queue was not drained.  The batch's tile tasks wait for the tiles of the previous step
+ * with bounded polling: at most max_polls polls (default 2^20, about one second) before
+ * the wait gives up, the run aborts and every workgroup exits.  unmet != 0 makes every
+ * such wait unsatisfiable -- a fault injection for testing that path from the host.  The
+ * next run recaptures its graph. */
+int iqsh_hea_set_wait_limit(iqsh_hea_handle h, uint32_t max_polls, int unmet);
 /* Final state of instance `inst` (2^n amplitudes of the batch dtype) after a run with
  * IQSH_HEA_KEEP_STATE. */
 int iqsh_hea_state(iqsh_hea_handle h, int inst, void *host);

@@ -93,6 +93,7 @@ template <typename T> struct HeaArgs {
   double* part;
   double* cpart;
   uint32_t spin_limit;
+  uint32_t wait_extra;  // added to every dependency target: 0, or 1 to make waits unsatisfiable
 };
 
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
@@ -543,7 +544,7 @@ __global__ __launch_bounds__((Tile<T, M, R>::NT), 2) void k_hea(HeaArgs<T> A)
         uint32_t ok = 1;
         if (tk.step > 0) {
           uint32_t* ctr = A.ctl + kCtlArrive + I.arrive + tk.step - 1;
-          const uint32_t need = 1u << I.ntl;
+          const uint32_t need = (1u << I.ntl) + A.wait_extra;
           uint32_t spins = 0;
           while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
             if (__hip_atomic_load(A.ctl + kCtlAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
@@ -569,6 +570,15 @@ __global__ __launch_bounds__((Tile<T, M, R>::NT), 2) void k_hea(HeaArgs<T> A)
         next = atomicAdd(head, 1u);
       }
       if (ok) {
+        // Publication of the tile: the compiler may not sink the state stores below the
+        // wait (the builtin alone does not order memory), the wait completes them in the
+        // XCD's L2, and the barrier puts every thread's wait before thread 0's arrival
+        // count.  This is the gfx950 form of a release at the L2 of one XCD: a register's
+        // tasks all run on one XCD (one queue per XCD), whose L2 is their point of
+        // coherence, and the consumer invalidates its L1 (acquire_tiles).  An agent-scope
+        // release fence here would also write back L2 for other XCDs, which no consumer
+        // needs; it made the 512-register run 2.6x slower.
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
         __builtin_amdgcn_s_waitcnt(0);  // this thread's state stores have completed
         __syncthreads();
         if (tid == 0)
@@ -577,6 +587,13 @@ __global__ __launch_bounds__((Tile<T, M, R>::NT), 2) void k_hea(HeaArgs<T> A)
       }
     }
   }
+}
+
+// Which XCDs serve workgroups: bit (XCC_ID % 8) of *mask for every workgroup that runs.
+__global__ void k_xcc_probe(uint32_t* mask)
+{
+  if (threadIdx.x == 0)
+    atomicOr(mask, 1u << (__builtin_amdgcn_s_getreg((20) | (0 << 6) | ((4 - 1) << 11)) % kQueues));
 }
 
 __global__ void k_hea_zero(uint32_t* p, uint32_t n)
@@ -692,6 +709,8 @@ struct iqsh_hea {
   uint64_t npairs = 0, arena_amps = 0;
   iqsh_hea_tune tune{0, 0, 0, 0};
   int cus = 0;
+  uint32_t qmask = 0xffu;     // queues whose XCD serves workgroups (k_xcc_probe)
+  uint32_t spin_limit = kSpinLimit, wait_extra = 0;
   hipStream_t cap = nullptr;  // private stream for graph capture
 
   void* arena = nullptr;
@@ -808,7 +827,10 @@ int build(iqsh_hea* h, int mode)
   for (int b = 0; b < h->count; ++b) all[b] = b;
   std::stable_sort(all.begin(), all.end(), [&](int a, int b) { return cost(a) > cost(b); });
   std::vector<int> qof(h->count);
-  double load[kQueues] = {0};
+  // Only queues whose XCD serves workgroups receive registers (a partitioned GPU exposes
+  // fewer XCDs).
+  double load[kQueues];
+  for (int qi = 0; qi < kQueues; ++qi) load[qi] = (h->qmask >> qi) & 1 ? 0.0 : HUGE_VAL;
   for (int b : all) {
     const int qi = (int)(std::min_element(load, load + kQueues) - load);
     qof[b] = qi;
@@ -922,7 +944,8 @@ int enqueue_t(iqsh_hea* h, int mode, hipStream_t st)
   a.cost = h->cost;
   a.part = h->part;
   a.cpart = h->cpart;
-  a.spin_limit = kSpinLimit;
+  a.spin_limit = h->spin_limit;
+  a.wait_extra = h->wait_extra;
   hipLaunchKernelGGL(k_hea_zero, dim3((h->nctl + 255) / 256), dim3(256), 0, st, h->ctl, h->nctl);
   IQSI_TRY(hipGetLastError());
   hipLaunchKernelGGL(k_hea_mats<T>, dim3((unsigned)((h->npairs + 255) / 256)), dim3(256), 0, st, (const double*)h->ang,
@@ -1026,6 +1049,21 @@ int iqsh_hea_create(int dtype, int count, const int* nq, const int* depth, const
     return bail(fail(IQSH_ERR_HIP, "clearing batch buffers failed"));
   if (hipStreamCreateWithFlags(&h->cap, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(IQSH_ERR_HIP, "hipStreamCreateWithFlags failed"));
+  {
+    // Probe which XCDs serve workgroups: 8 workgroups per CU reach every XCD, since
+    // workgroups are dispatched round-robin over them.
+    uint32_t* m = nullptr;
+    if ((rc = alloc((void**)&m, sizeof(uint32_t), "XCD probe"))) return bail(rc);
+    uint32_t found = 0;
+    bool okp = hipMemset(m, 0, sizeof(uint32_t)) == hipSuccess;
+    if (okp) {
+      hipLaunchKernelGGL(k_xcc_probe, dim3(8 * h->cus), dim3(64), 0, nullptr, m);
+      okp = hipGetLastError() == hipSuccess && hipMemcpy(&found, m, sizeof found, hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    (void)hipFree(m);
+    if (!okp || !found) return bail(fail(IQSH_ERR_HIP, "XCD probe failed"));
+    h->qmask = found;
+  }
   if ((rc = build(h, kGrad))) return bail(rc);
   *out = h;
   return IQSH_OK;
@@ -1129,6 +1167,28 @@ int iqsh_hea_run(iqsh_hea_handle h, int flags)
   return IQSH_OK;
 }
 
+namespace {
+int check_ctl(const iqsh_hea* h, const uint32_t* ctl)
+{
+  if (ctl[kCtlAbort]) return fail(IQSH_ERR_HIP, "batch run aborted: a tile dependency wait timed out");
+  for (int qi = 0; qi < kQueues; ++qi)
+    if (ctl[qi] < h->qlen[qi])
+      return fail(IQSH_ERR_HIP, "batch run incomplete: queue %d served %u of %u tasks (no workgroup on its XCD)", qi,
+                  ctl[qi], h->qlen[qi]);
+  return IQSH_OK;
+}
+}  // namespace
+
+int iqsh_hea_set_wait_limit(iqsh_hea_handle h, uint32_t max_polls, int unmet)
+{
+  if (int rc = check_handle(h)) return rc;
+  if (max_polls < 1) return fail(IQSH_ERR_ARG, "max_polls must be at least 1");
+  h->spin_limit = max_polls;
+  h->wait_extra = unmet ? 1u : 0u;
+  drop_graph(h);  // the captured launch holds the old limits
+  return IQSH_OK;
+}
+
 int iqsh_hea_results(iqsh_hea_handle h, double* cost, double* grad)
 {
   if (int rc = check_handle(h)) return rc;
@@ -1138,18 +1198,19 @@ int iqsh_hea_results(iqsh_hea_handle h, double* cost, double* grad)
   if (cost) IQSI_TRY(hipMemcpyAsync(cost, h->cost, (size_t)h->count * sizeof(double), hipMemcpyDeviceToHost, st));
   if (grad) IQSI_TRY(hipMemcpyAsync(grad, h->grad, h->npairs * 2 * sizeof(double), hipMemcpyDeviceToHost, st));
   IQSI_TRY(hipStreamSynchronize(st));
-  if (ctl[kCtlAbort]) return fail(IQSH_ERR_HIP, "batch run aborted: a tile dependency wait timed out");
-  for (int qi = 0; qi < kQueues; ++qi)
-    if (ctl[qi] < h->qlen[qi])
-      return fail(IQSH_ERR_HIP, "batch run incomplete: queue %d served %u of %u tasks (no workgroup on its XCD)", qi,
-                  ctl[qi], h->qlen[qi]);
-  return IQSH_OK;
+  return check_ctl(h, ctl);
 }
 
 int iqsh_hea_results_device(iqsh_hea_handle h, double* cost, double* grad)
 {
   if (int rc = check_handle(h)) return rc;
   hipStream_t st = iqsi::stream();
+  // The run's control words decide whether the results are valid: read them first (this
+  // synchronizes the stream, as the host form does).
+  uint32_t ctl[kCtlArrive] = {0};
+  IQSI_TRY(hipMemcpyAsync(ctl, h->ctl, sizeof ctl, hipMemcpyDeviceToHost, st));
+  IQSI_TRY(hipStreamSynchronize(st));
+  if (int rc = check_ctl(h, ctl)) return rc;
   if (cost) IQSI_TRY(hipMemcpyAsync(cost, h->cost, (size_t)h->count * sizeof(double), hipMemcpyDeviceToDevice, st));
   if (grad) IQSI_TRY(hipMemcpyAsync(grad, h->grad, h->npairs * 2 * sizeof(double), hipMemcpyDeviceToDevice, st));
   return IQSH_OK;

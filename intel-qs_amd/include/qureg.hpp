@@ -13,8 +13,18 @@
 /// Differences from the reference that a caller can observe are documented at each method.
 #pragma once
 
+// The reference's qureg.hpp includes these standard headers (include/qureg.hpp:6-17), and
+// its callers rely on them without including them.
+#include <algorithm>
 #include <cassert>
+#include <cmath>
+#include <fstream>
+#include <functional>
+#include <iostream>
+#include <limits>
+#include <map>
 #include <memory>
+#include <numeric>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -319,3 +329,8 @@ TM4x4<Type> ChannelBranchMatrix(CM16x16<Type>& chi, RandomNumberGenerator<typena
 // (the reference includes both here too, include/qureg.hpp:429-433).
 #include "QubitRegisterMetric.hpp"
 #include "NoisyQureg.hpp"
+
+// The reference's QubitRegisterMetric.hpp and NoisyQureg.hpp, included at the end of its
+// qureg.hpp, declare `using namespace std;` at file scope, and callers depend on it
+// (benchmarks/basic_code_for_scaling.cpp:156-157 uses ofstream and ios unqualified).
+using namespace std;

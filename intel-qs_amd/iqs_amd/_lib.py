@@ -94,6 +94,7 @@ def _sig(L):
         "iqsh_hea_run": [P, I],
         "iqsh_hea_results": [P, P, P],
         "iqsh_hea_results_device": [P, P, P],
+        "iqsh_hea_set_wait_limit": [P, C.c_uint32, I],
         "iqsh_hea_state": [P, I, P],
         "iqsh_hea_traffic": [P, I, DP],
         "iqsh_hea_autotune": [P, I, I, P, I, C.POINTER(I), DP],

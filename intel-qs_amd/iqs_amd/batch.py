@@ -149,7 +149,14 @@ class HeaBatch:
         return (cost, g) if grad else cost
 
     def results_device(self, cost_ptr: int, grad_ptr: Optional[int]) -> None:
+        """Copy costs and gradients into device memory; raises (copying nothing) if the run
+        aborted or left a task queue undrained."""
         call("iqsh_hea_results_device", self.h, C.c_void_p(cost_ptr), C.c_void_p(grad_ptr) if grad_ptr else None)
+
+    def set_wait_limit(self, max_polls: int = 1 << 20, unmet: bool = False) -> None:
+        """Bounded dependency waits: polls before a tile wait gives up and the run reports an
+        error; unmet=True makes every wait unsatisfiable (fault injection for tests)."""
+        call("iqsh_hea_set_wait_limit", self.h, int(max_polls), int(bool(unmet)))
 
     def state(self, b: int) -> np.ndarray:
         dt = np.complex64 if self.dtype == "c64" else np.complex128

@@ -48,10 +48,8 @@ def run(c: Circuit, reg=None) -> Tuple[object, List[str]]:
     def ensure_diag():
         nonlocal diag
         if diag is None:
-            if c.precision == "sp":
-                raise NotImplementedError("the module's QAOA functions are bound for ComplexDP registers")
             n = c.num_qubits
-            diag = m.QubitRegister(n, "base", 0, 0)
+            diag = (m.QubitRegisterSP if c.precision == "sp" else m.QubitRegister)(n, "base", 0, 0)
             weighted = any(w != 1.0 for (_, _, w) in c.graph)
             if weighted:
                 adj = [0.0] * (n * n)

@@ -444,6 +444,21 @@ PYBIND11_MODULE(intelqs_py, m)
   m.def("GetExpectationValueSquaredFromCostFunction",
         &iqs::qaoa::GetExpectationValueSquaredFromCostFunction<ComplexDP>,
         "Get expectation value squared from the cost function.");
+  // The reference module binds these for ComplexDP registers only (pybind11/intelqs_py.cpp:
+  // 359-391); its C++ templates accept both precisions (src/qaoa_features.cpp), so the same
+  // names also take QubitRegisterSP registers (pybind11 picks the overload by type).
+  m.def("InitializeVectorAsMaxCutCostFunction", &iqs::qaoa::InitializeVectorAsMaxCutCostFunction<ComplexSP>);
+  m.def("InitializeVectorAsWeightedMaxCutCostFunction",
+        &iqs::qaoa::InitializeVectorAsWeightedMaxCutCostFunction<ComplexSP>);
+  m.def("ImplementQaoaLayerBasedOnCostFunction", &iqs::qaoa::ImplementQaoaLayerBasedOnCostFunction<ComplexSP>);
+  m.def("GetExpectationValueFromCostFunction", &iqs::qaoa::GetExpectationValueFromCostFunction<ComplexSP>);
+  m.def("GetHistogramFromCostFunction", &iqs::qaoa::GetHistogramFromCostFunction<ComplexSP>);
+  m.def("GetHistogramFromCostFunctionWithWeightsRounded",
+        &iqs::qaoa::GetHistogramFromCostFunctionWithWeightsRounded<ComplexSP>);
+  m.def("GetHistogramFromCostFunctionWithWeightsBinned",
+        &iqs::qaoa::GetHistogramFromCostFunctionWithWeightsBinned<ComplexSP>);
+  m.def("GetExpectationValueSquaredFromCostFunction",
+        &iqs::qaoa::GetExpectationValueSquaredFromCostFunction<ComplexSP>);
 
   py::class_<Environment>(m, "MPIEnvironment")
       .def(py::init<>())

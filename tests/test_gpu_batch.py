@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import oracle
+from iqs_amd._lib import IqsError
 from iqs_amd.batch import HeaBatch, instance_circuit, param_offsets, random_batch, torch_function
 
 pytestmark = pytest.mark.gpu
@@ -164,7 +165,8 @@ def test_full_batch_512_instances():
     c64, g64 = b64.evaluate(ang)
     ec, eg = np.max(np.abs(c64 - c128)), np.max(np.abs(g64 - g128))
     print(f"512 instances: cost err {ec:.2e}  grad err {eg:.2e}")
-    assert ec < 1e-3 and eg < 1e-3
+    # measured: cost 9.8e-7, gradient 1.9e-6 (FP32 amplitudes, FP64 sums)
+    assert ec < 1e-5 and eg < 1e-5
     b64.close()
     b128.close()
 
@@ -183,4 +185,35 @@ def test_torch_function_backward():
     assert np.allclose(cost.detach().cpu().numpy(), c_ref, atol=0, rtol=0)
     scale = np.repeat(np.arange(1, b.count + 1, dtype=np.float64), np.diff(b.offsets))
     assert np.array_equal(a.grad.cpu().numpy(), g_ref * scale)
+    b.close()
+
+
+def test_dependency_wait_gives_up_cleanly():
+    """Fault injection from the host: every tile dependency wait is made unsatisfiable with
+    a 64-poll limit.  The run must end (every workgroup exits), both result forms must
+    report the abort and copy nothing, the torch Function must raise; restoring the
+    default limit gives the normal results again."""
+    import torch
+
+    nq, dp, ang = small_batch(5)
+    b = HeaBatch(nq, dp, "c128", tune=(TEST_VARIANT, 0, 0))
+    c0, g0 = b.evaluate(ang)
+    b.set_wait_limit(64, unmet=True)
+    b.set_angles(ang)
+    b.run(grad=True)
+    with pytest.raises(IqsError, match="aborted"):
+        b.results()
+    cost = torch.full((b.count,), float("nan"), dtype=torch.float64, device="cuda")
+    grad = torch.full((b.num_params,), float("nan"), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    with pytest.raises(IqsError, match="aborted"):
+        b.results_device(cost.data_ptr(), grad.data_ptr())
+    assert torch.isnan(cost).all() and torch.isnan(grad).all()
+    f = torch_function()
+    a = torch.tensor(ang, dtype=torch.float64, device="cuda", requires_grad=True)
+    with pytest.raises(IqsError, match="aborted"):
+        f.apply(a, b)
+    b.set_wait_limit()
+    c, g = b.evaluate(ang)
+    assert np.array_equal(c, c0) and np.array_equal(g, g0)
     b.close()

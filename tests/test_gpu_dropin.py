@@ -41,8 +41,6 @@ def compare_results(got, want, prec):
 @pytest.mark.parametrize("name", CASES)
 def test_fixture_through_dropin(name):
     c, want_res, g = load(name)
-    if c.precision == "sp" and c.graph:
-        pytest.skip("the module binds the QAOA functions for ComplexDP registers, as the reference module does")
     reg, res = runner.run(c)
     st = runner.state_of(reg)
     if "state" in g.files:

@@ -54,6 +54,30 @@ def apply_op(sv, op, G):
         raise ValueError(name)
 
 
+def apply_inverse_op(sv, op, G):
+    """The inverse of apply_op's gate: H and CNOT are involutions, R(theta)^-1 = R(-theta)."""
+    name = op[0]
+    if name in G.ROT_1Q:
+        sv.apply_1q(op[1], G.ROT_1Q[name](-op[2]))
+    else:
+        apply_op(sv, op, G)
+
+
+def host_info() -> dict:
+    """CPU model, logical CPUs and physical cores of this host (lscpu)."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=20).stdout
+        cores = {tuple(l.split(",")) for l in out.splitlines() if l and not l.startswith("#")}
+        info["physical_cores"] = len(cores)
+        for l in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout.splitlines():
+            if l.startswith("Model name:"):
+                info["model"] = l.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
 def kernel_kind(op):
     """Which kernel a brick-circuit gate launches (iqs_capi.hip dispatch)."""
     name = op[0]
@@ -88,6 +112,101 @@ def calibrate(n: int) -> int:
     return 0
 
 
+def _time_ms(fn, reps, stream, torch):
+    """Average milliseconds of fn() over reps calls (one warm-up), HIP events on `stream`."""
+    fn()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+        fn()
+    b.record(stream)
+    b.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def _rates(rows, nbytes):
+    ts = [t for _, t in rows]
+    return {"bytes_per_call": nbytes, "avg_ms": sum(ts) / len(ts),
+            "min_GBs": nbytes / max(ts) / 1e6, "avg_GBs": nbytes / (sum(ts) / len(ts)) / 1e6,
+            "max_GBs": nbytes / min(ts) / 1e6, "per_case_GBs": [round(nbytes / t / 1e6, 1) for _, t in rows]}
+
+
+def kind_sweep(sv, n: int, esz: int, reps: int, stream, torch):
+    """GB/s per gate kind at every target position (the reference's byte model, Timer
+    bookkeeping of src/highperfkernels.cpp:230-231, 370, 478-479: a dense update reads and
+    writes the state, 2S; a controlled gate or a swap touches half of it, S)."""
+    import numpy as np
+    from iqs_amd import gates as G
+    from iqs_amd.circuits import SplitMix64, haar_unitary, random_su2
+
+    S = (1 << n) * esz
+    srng = SplitMix64(7)
+    kinds = {
+        "1q_dense": ([(p,) for p in range(n)], lambda p: sv.apply_1q(p[0], random_su2(srng)), 2 * S),
+        "controlled_1q": ([((t + 1) % n, t) for t in range(n)] + [(0, n - 1), (n - 1, 0)],
+                          lambda p: sv.apply_c1q(p[0], p[1], random_su2(srng)), S),
+        "diag_2q": ([(p, (p + 1) % n) for p in range(n)],
+                    lambda p: sv.apply_diag2(p[0], p[1], np.exp(1j * np.array([srng.uniform(0, 6) for _ in range(4)]))),
+                    2 * S),
+        "swap": ([(p, n - 1 - p) for p in range(n // 2)], lambda p: sv.apply_swaplike(p[0], p[1], G.SWAP), S),
+        "dense_2q": ([(p, p + 1) for p in range(n - 1)] + [(0, n - 1)],
+                     lambda p: sv.apply_2q(p[0], p[1], haar_unitary(4, srng)), 2 * S),
+    }
+    out = {}
+    for kname, (targets, fn, nbytes) in kinds.items():
+        rows = [(t, _time_ms(lambda: fn(t), reps, stream, torch)) for t in targets]
+        out[kname] = dict(_rates(rows, nbytes), cases=[list(t) for t in targets])
+        log(f"sweep {kname} ({n} qubits, {esz} B): {out[kname]['avg_GBs']:.0f} GB/s average")
+    return out
+
+
+def permute_sweep(sv, n: int, esz: int, reps: int, stream, torch):
+    """PermuteQubits (src/qureg_permute.cpp:57-105) as one bit-exact index permutation:
+    reads and writes the state once (2S).  Each case is applied and then undone, so the
+    register ends where it started."""
+    import numpy as np
+
+    S = (1 << n) * esz
+    cases = {
+        "reversal": list(reversed(range(n))),
+        "rotate_up_1": [(b + 1) % n for b in range(n)],
+        "rotate_down_1": [(b - 1) % n for b in range(n)],
+        "swap_halves": [(b + n // 2) % n for b in range(n)],
+        "swap_0_top": [n - 1] + list(range(1, n - 1)) + [0],
+        "random": [int(x) for x in np.random.default_rng(n).permutation(n)],
+    }
+    rows = []
+    for name, perm in cases.items():
+        inv = [0] * n
+        for b, src in enumerate(perm):
+            inv[src] = b
+        ms = _time_ms(lambda: (sv.permute(perm), sv.permute(inv)), reps, stream, torch) / 2
+        rows.append((name, ms))
+    out = _rates(rows, 2 * S)
+    out["cases"] = {k: v for k, v in cases.items()}
+    log(f"sweep PermuteQubits ({n} qubits, {esz} B): {out['avg_GBs']:.0f} GB/s average")
+    return out
+
+
+def reduction_sweep(sv, n: int, esz: int, reps: int, stream, torch):
+    """FP64 reductions: GetProbability at every position (reads the half with the bit set,
+    S/2; src/qureg_measure.cpp:136-178), the norm (S) and one-pass Pauli expectations with
+    X, Y and Z factors (S; replaces src/qureg_expectval.cpp:102-213)."""
+    S = (1 << n) * esz
+    prob = [(p, _time_ms(lambda: sv.prob1(p), reps, stream, torch)) for p in range(n)]
+    H = n - 1
+    paulis = {"Z0": (0, 1), "ZtopZ0": (0, 1 | (1 << H)), "X0": (1, 0), "Xtop": (1 << H, 0),
+              "Y0Y1": (3, 3), "X0Ztop": (1, 1 << H), "XYZ_spread": ((1 << 2) | (1 << (H - 3)), (1 << (H - 3)) | (1 << 7))}
+    pv = [(k, _time_ms(lambda: sv.expval_pauli(*m), reps, stream, torch)) for k, m in paulis.items()]
+    out = {"prob1": _rates(prob, S // 2),
+           "norm2": _rates([("norm", _time_ms(sv.norm2, reps, stream, torch))], S),
+           "expval_pauli": dict(_rates(pv, S), cases=list(paulis))}
+    log(f"sweep reductions ({n} qubits, {esz} B): prob1 {out['prob1']['avg_GBs']:.0f}, "
+        f"expval {out['expval_pauli']['avg_GBs']:.0f} GB/s average")
+    return out
+
+
 def cpu_baseline(n: int, layer_ops, budget_s: float = 30.0):
     """Time the compiled reference on layer 0 of the same circuit (bounded by budget_s)."""
     drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
@@ -112,7 +231,8 @@ def cpu_baseline(n: int, layer_ops, budget_s: float = 30.0):
         return None
     return {"value": len(times) / sum(times), "unit": "gates/s", "cores": threads, "kind": "reference",
             "sample": f"{n}-qubit ComplexDP brick layer 0 ({len(times)} gates) through the compiled reference "
-                      f"(intel-qs src/, -O3 OpenMP), per-gate steady_clock"}
+                      f"(intel-qs src/, -O3 OpenMP, OMP_PROC_BIND=spread OMP_PLACES=cores), per-gate steady_clock",
+            "host": host_info()}
 
 
 def brick20_against_reference(torch):
@@ -269,6 +389,26 @@ def hea_sample_on_reference(torch, count: int = 24, max_qubits: int = 20, budget
             "max_abs_cost_diff_gpu_c128_vs_reference": float(np.max(np.abs(np.array(gpu_cost) - np.array(ref_cost))))}
 
 
+def state_error(a, b):
+    """(1 - |<b|a>|^2 / (<a|a><b|b>), max |a_i - b_i|) of GPU state a against the complex128
+    GPU state b of the same size; a complex64 state is widened into a scratch register."""
+    from iqs_amd._lib import call
+    from iqs_amd.engine import StateVector
+    import numpy as np
+    tmp = None
+    if a.np_dtype != np.complex128:
+        tmp = StateVector(a.num_qubits, "c128")
+        call("iqsh_convert", tmp.handle, a.handle)
+        a = tmp
+    try:
+        ov = a.overlap(b)
+        f = abs(ov) ** 2 / (a.norm2() * b.norm2())
+        return 1.0 - f, a.maxabsdiff(b)
+    finally:
+        if tmp is not None:
+            tmp.close()
+
+
 def qaoa_measure(n: int, p: int, full: bool, torch):
     """28-qubit QAOA MaxCut gradient: adjoint vs parameter-shift on the same GPU."""
     import numpy as np
@@ -302,6 +442,8 @@ def qaoa_measure(n: int, p: int, full: bool, torch):
     c2, dg2, db2 = p64.adjoint(g64, b64)
     torch.cuda.synchronize()
     t_adj64 = time.perf_counter() - t0
+    # final complex64 state against the complex128 one (FP64 reductions on the GPU)
+    inf64, mx64 = state_error(p64.forward_state(g64, b64), prob.forward_state(g, b))
     return {
         "qubits": n, "layers": p, "edges": len(prob.edges), "params": 2 * p, "dtype": "f64",
         "cost": cost, "forward_s": t_fwd, "adjoint_grad_s": t_adj,
@@ -312,7 +454,8 @@ def qaoa_measure(n: int, p: int, full: bool, torch):
         "max_abs_diff_adjoint_vs_param_shift": diff,
         "central_fd_h1e-4_gamma_last": fd, "adjoint_gamma_last": dg[-1],
         "c64": {"adjoint_grad_s": t_adj64, "cost_abs_err_vs_f64": abs(c2 - cost),
-                "grad_max_abs_err_vs_f64": float(max(np.abs(dg2 - dg).max(), np.abs(db2 - db).max()))},
+                "grad_max_abs_err_vs_f64": float(max(np.abs(dg2 - dg).max(), np.abs(db2 - db).max())),
+                "state_one_minus_fidelity_vs_f64": inf64, "state_max_abs_err_vs_f64": mx64},
     }
 
 
@@ -426,12 +569,25 @@ def batch_measure(reps: int, torch):
     out["c64"] = {"ms_forward_and_gradient": ms_grad, "ms_forward_only": ms_fwd,
                   "instances_per_s": len(nq) / (ms_grad * 1e-3),
                   "algorithmic_bytes": traffic, "algorithmic_GBs": traffic / (ms_grad * 1e-3) / 1e9}
-    b64.close()
     log("HEA batch: complex128 reference run")
     b128 = HeaBatch(nq, dp, "c128")
     c128, g128 = b128.evaluate(ang)
     out["c128"] = {"ms_forward_and_gradient": timed(lambda: b128.run(grad=True), max(1, reps // 2))}
     ps = b128.parameter_shift(ang, [0] * len(nq))
+    # final states of every instance, complex64 against complex128
+    b64.run(grad=False, keep_state=True)
+    b128.set_angles(ang)
+    b128.run(grad=False, keep_state=True)
+    inf, mx = [], []
+    for i in range(len(nq)):
+        a, r = b64.state(i).astype(np.complex128), b128.state(i)
+        f = abs(np.vdot(r, a)) ** 2 / (np.vdot(a, a).real * np.vdot(r, r).real)
+        inf.append(1.0 - f)
+        mx.append(float(np.abs(a - r).max()))
+    out["c64"].update({"state_one_minus_fidelity_vs_f64_max": float(max(inf)),
+                       "state_one_minus_fidelity_vs_f64_median": float(np.median(inf)),
+                       "state_max_abs_err_vs_f64": float(max(mx))})
+    b64.close()
     b128.close()
     out["c128"]["grad_vs_param_shift_max_abs"] = float(np.max(np.abs(g128[np.cumsum([0] + [2 * n * d for n, d in zip(nq, dp)])[:-1]] - ps)))
     out["c64"].update({"cost_max_abs_err_vs_f64": float(np.max(np.abs(c64 - c128))),
@@ -447,11 +603,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--qubits", type=int, default=30)
     ap.add_argument("--sweep-reps", type=int, default=3)
+    ap.add_argument("--sweep-c64-qubits", type=int, default=32, help="complex64 sweep register (0 = none)")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true", help="one 2^28-amplitude complex128 copy, for PMC scaling")
     ap.add_argument("--no-qaoa", action="store_true")
-    ap.add_argument("--qaoa-full", action="store_true", help="run all parameter-shift evaluations (minutes)")
+    ap.add_argument("--qaoa-sample", action="store_true",
+                    help="parameter-shift for gamma_1 and beta_1 only (the full run takes about 100 s)")
     ap.add_argument("--qaoa-qubits", type=int, default=28)
     ap.add_argument("--no-fusion", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -476,7 +634,7 @@ def main() -> int:
         return 1
     from iqs_amd import gates as G
     from iqs_amd._lib import call
-    from iqs_amd.circuits import SplitMix64, brick_layer_ops, haar_unitary, random_su2
+    from iqs_amd.circuits import SplitMix64, brick_layer_ops
     from iqs_amd.engine import StateVector
 
     torch.cuda.init()
@@ -528,47 +686,34 @@ def main() -> int:
     readout = [1.0 - 2.0 * sv.prob1(q) for q in range(n)]
     log(f"timed brick circuit: {ngates} gates in {elapsed:.2f} s")
 
-    # ---- per-kind sweep over every position ----
-    sweep = {}
-    if not args.no_sweep:
-        srng = SplitMix64(7)
-        kinds = {
-            "1q_dense": ([(p,) for p in range(n)], lambda p: sv.apply_1q(p[0], random_su2(srng)), 2 * S),
-            "controlled_1q": ([((t + 1) % n, t) for t in range(n)] + [(0, n - 1), (n - 1, 0)],
-                              lambda p: sv.apply_c1q(p[0], p[1], random_su2(srng)), S),
-            "diag_2q": ([(p, (p + 1) % n) for p in range(n)],
-                        lambda p: sv.apply_diag2(p[0], p[1], np.exp(1j * np.array([srng.uniform(0, 6) for _ in range(4)]))), 2 * S),
-            "swap": ([(p, n - 1 - p) for p in range(n // 2)], lambda p: sv.apply_swaplike(p[0], p[1], G.SWAP), S),
-            "dense_2q": ([(p, p + 1) for p in range(n - 1)] + [(0, n - 1)],
-                         lambda p: sv.apply_2q(p[0], p[1], haar_unitary(4, srng)), 2 * S),
-        }
-        for kname, (targets, fn, nbytes) in kinds.items():
-            rows = []
-            for tgt in targets:
-                fn(tgt)  # warm-up
-                a = torch.cuda.Event(enable_timing=True)
-                b = torch.cuda.Event(enable_timing=True)
-                a.record(stream)
-                for _ in range(args.sweep_reps):
-                    fn(tgt)
-                b.record(stream)
-                b.synchronize()
-                t = a.elapsed_time(b) * 1e-3 / args.sweep_reps
-                rows.append((tgt, t))
-            ts = [t for _, t in rows]
-            sweep[kname] = {
-                "bytes_per_gate": nbytes,
-                "avg_ms": 1e3 * sum(ts) / len(ts),
-                "min_GBs": nbytes / max(ts) / 1e9,
-                "avg_GBs": nbytes / (sum(ts) / len(ts)) / 1e9,
-                "max_GBs": nbytes / min(ts) / 1e9,
-                "per_target_GBs": [round(nbytes / t / 1e9, 1) for _, t in rows],
-            }
-            log(f"sweep {kname}: {sweep[kname]['avg_GBs']:.0f} GB/s average")
+    # ---- the same circuit undone: distance back to |0...0> (FP64 error of the round trip) ----
+    for ops in reversed(layers):
+        for op in reversed(ops):
+            apply_inverse_op(sv, op, G)
+    ref0 = StateVector(n, "c128")
+    ref0.init_basis(0)
+    roundtrip = sv.maxabsdiff(ref0)
+    ref0.close()
+    log(f"brick circuit undone: max |psi - |0>| = {roundtrip:.3e}")
 
+    # ---- per-kind, permutation and reduction sweeps over every position ----
+    sweep = sweep_c64 = perm = red = None
+    if not args.no_sweep:
+        sweep = kind_sweep(sv, n, 16, args.sweep_reps, stream, torch)
+        perm = permute_sweep(sv, n, 16, args.sweep_reps, stream, torch)
+        red = reduction_sweep(sv, n, 16, args.sweep_reps, stream, torch)
     del sv
+    if not args.no_sweep and args.sweep_c64_qubits:
+        n64 = args.sweep_c64_qubits
+        sv64 = StateVector(n64, "c64")
+        sv64.init_uniform(1.0 / math.sqrt(float(1 << n64)))
+        sweep_c64 = kind_sweep(sv64, n64, 8, args.sweep_reps, stream, torch)
+        sweep_c64["permute"] = permute_sweep(sv64, n64, 8, args.sweep_reps, stream, torch)
+        sweep_c64["reductions"] = reduction_sweep(sv64, n64, 8, args.sweep_reps, stream, torch)
+        sweep_c64["qubits"] = n64
+        sv64.close()
     log("brick circuit and per-kind sweep done")
-    qaoa = None if args.no_qaoa else qaoa_measure(args.qaoa_qubits, 8, args.qaoa_full, torch)
+    qaoa = None if args.no_qaoa else qaoa_measure(args.qaoa_qubits, 8, not args.qaoa_sample, torch)
     log("QAOA gradient done")
     fusion = None if args.no_fusion else fusion_measure(args.fusion_qubits, 200, torch)
     log("fusion done")
@@ -607,7 +752,11 @@ def main() -> int:
         "hbm_frac_1q_all_targets": (2 * S / (sum(dense + per_kind.get("dense1q_lane", [])) /
                                              len(dense + per_kind.get("dense1q_lane", []))) / 1e9) / HBM_PEAK_GBS,
         "z_readout_first4": readout[:4],
+        "brick_roundtrip_max_abs_err": roundtrip,
         "sweep": sweep,
+        "sweep_permute": perm,
+        "sweep_reductions": red,
+        "sweep_c64": sweep_c64,
         "qaoa_grad": qaoa,
         "fusion": fusion,
         "hea_batch": batch,

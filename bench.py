@@ -235,6 +235,56 @@ def cpu_baseline(n: int, layer_ops, budget_s: float = 30.0):
             "host": host_info()}
 
 
+def cpu_kind_sample(n: int):
+    """The per-gate-kind sweep's byte model on the compiled reference at n qubits: positions
+    low, middle and high per kind, each gate applied twice (the first run is a warm-up),
+    except the reference's serial kinds (ApplyDiag, Apply2QubitGate run on one thread, about
+    1 GB/s): one middle position, applied once.  GB/s from the driver's per-gate timer."""
+    import numpy as np
+    from iqs_amd.circuits import Circuit, SplitMix64, haar_unitary, random_su2
+
+    drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if not os.path.exists(drv):
+        return None
+    r = SplitMix64(11)
+    H, M = n - 1, n // 2
+    S = (1 << n) * 16
+    kinds = [
+        ("1q_dense", 2 * S, 2, [("Apply1QubitGate", p, random_su2(r)) for p in (0, M, H)]),
+        ("controlled_1q", S, 2, [("ApplyControlled1QubitGate", c, t, random_su2(r)) for c, t in ((1, 0), (M + 1, M), (0, H))]),
+        ("swap", S, 2, [("ApplySwap", a, b) for a, b in ((0, H), (7, H - 7), (M - 1, M))]),
+        ("diag_2q", 2 * S, 1, [("ApplyDiag", M, M + 1, np.exp(1j * np.array([r.uniform(0, 6) for _ in range(4)])))]),
+        ("dense_2q", 2 * S, 1, [("Apply2QubitGate", M + 1, M, haar_unitary(4, r))]),
+    ]
+    c = Circuit(n)
+    plan = []
+    for name, nbytes, rep, ops in kinds:
+        for op in ops:
+            for k in range(rep):
+                c.add(*op)
+                plan.append((name, nbytes, k == rep - 1))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    with tempfile.TemporaryDirectory() as d:
+        cf = os.path.join(d, "c.txt")
+        c.write(cf)
+        try:
+            res = subprocess.run([drv, cf, "--time"], capture_output=True, text=True, timeout=600,
+                                 env=dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="spread",
+                                          OMP_PLACES="cores"))
+        except subprocess.TimeoutExpired:
+            return None
+    times = [float(l.split()[3]) for l in res.stderr.splitlines() if l.startswith("T ")]
+    if len(times) != len(plan):
+        return None
+    out = {}
+    for (name, nbytes, timed), t in zip(plan, times):
+        if timed:
+            out.setdefault(name, []).append(round(nbytes / t / 1e9, 2))
+    return {"qubits": n, "threads": threads, "GBs_per_case": out,
+            "cases": "positions low / middle / high per kind; ApplyDiag and Apply2QubitGate (serial in the "
+                     "reference) at the middle position only"}
+
+
 def brick20_against_reference(torch):
     """The reference's CPU-runnable case: a 20-qubit ComplexDP brick circuit of depth 20
     (seed 20240601) with <Z_i> read-outs, run through the drop-in on the GPU and through the
@@ -459,6 +509,40 @@ def qaoa_measure(n: int, p: int, full: bool, torch):
     }
 
 
+def brick_fused_measure(sv, layers, torch):
+    """The timed brick circuit again through the drop-in with fusion on (TurnOnFusion,
+    <= 5-position blocks on the matrix cores): gates/s of the whole queued-and-flushed run
+    and its distance from the unfused state `sv` (complex128, same gates)."""
+    import ctypes as C
+
+    import intelqs_py as iqs
+    from iqs_amd import runner
+    from iqs_amd._lib import call
+    from iqs_amd.circuits import Circuit
+
+    n = sv.num_qubits
+    c = Circuit(n)
+    for ops in layers:
+        for op in ops:
+            c.add(*op)
+    reg = iqs.QubitRegister(n, "base", 0, 0)
+    reg.SetFusionBlockQubits(5)
+    reg.TurnOnFusion()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    runner.run(c, reg)  # ends with TurnOffFusion: every queued block applied
+    reg.Sync()
+    t = time.perf_counter() - t0
+    gates, blocks = reg.FusionStats()
+    mx = C.c_double()
+    call("iqsh_maxabsdiff", C.c_void_p(reg.DeviceHandle()), sv.handle, 1.0, 0.0, C.byref(mx))
+    del reg
+    return {"workload": f"the timed {n}-qubit brick layers ({len(c.ops)} gates) through the drop-in with "
+                        f"TurnOnFusion, <= 5-position blocks; not the headline value (that is one pass per gate)",
+            "gates": gates, "blocks": blocks, "s": t, "gates_per_s": gates / t,
+            "max_abs_diff_vs_unfused": mx.value}
+
+
 def fusion_measure(n: int, ngates: int, torch):
     """Random Haar 1q/2q circuit (seed 20240603) fused into <= 5-qubit blocks through the
     drop-in (complex64 and complex128), against the FP64 unfused run of the same gates."""
@@ -529,7 +613,7 @@ def fusion_measure(n: int, ngates: int, torch):
     return out
 
 
-def batch_measure(reps: int, torch):
+def batch_measure(reps: int, torch, tune=None):
     """512 hardware-efficient-ansatz instances (n ~ U{10..22}, d ~ U{10..50}, seed 20240606):
     forward + adjoint gradient of every angle in one persistent, graph-replayed launch.
     complex64 (autotuned) timed; complex128 run of the same batch as the FP64 reference;
@@ -543,11 +627,15 @@ def batch_measure(reps: int, torch):
            "params": int(2 * sum(n * d for n, d in zip(nq, dp)))}
     b64 = HeaBatch(nq, dp, "c64")
     b64.set_angles(ang)
-    log("HEA batch: autotuning complex64")
-    t0 = time.perf_counter()
-    tune, best_ms, trials = b64.autotune(grad=True, reps=2)
-    log(f"HEA batch: tuned {tune} at {best_ms:.2f} ms")
-    out["autotune_s"] = time.perf_counter() - t0
+    if tune is None:
+        log("HEA batch: autotuning complex64")
+        t0 = time.perf_counter()
+        tune, best_ms, trials = b64.autotune(grad=True, reps=2)
+        log(f"HEA batch: tuned {tune} at {best_ms:.2f} ms")
+        out["autotune_s"] = time.perf_counter() - t0
+    else:  # a given tuning (profiling runs): no autotune trials
+        b64.tune = tune
+        tune, trials = b64.tune, []
     out["tune"] = {"variant": tune[0], "workers_per_cu": tune[1], "order": tune[2], "group_mib": tune[3],
                    "tile": HeaBatch.variants("c64")[tune[0]]}
     out["trials_ms"] = [[list(t), round(ms, 3)] for t, ms in trials]
@@ -616,6 +704,8 @@ def main() -> int:
     ap.add_argument("--batch-only", action="store_true",
                     help="only the HEA batch section (for profiling its kernels); no metric value")
     ap.add_argument("--batch-reps", type=int, default=5)
+    ap.add_argument("--batch-tune", default="",
+                    help="variant,workers_per_cu,order,group_mib: skip the autotuner (profiling runs)")
     ap.add_argument("--fusion-qubits", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -641,8 +731,9 @@ def main() -> int:
     stream = torch.cuda.current_stream()
     call("iqsh_set_stream", stream.cuda_stream)
 
+    batch_tune = tuple(int(x) for x in args.batch_tune.split(",")) if args.batch_tune else None
     if args.batch_only:
-        print(json.dumps({"hea_batch": batch_measure(args.batch_reps, torch)}))
+        print(json.dumps({"hea_batch": batch_measure(args.batch_reps, torch, batch_tune)}))
         return 0
 
     n = args.qubits
@@ -685,6 +776,9 @@ def main() -> int:
 
     readout = [1.0 - 2.0 * sv.prob1(q) for q in range(n)]
     log(f"timed brick circuit: {ngates} gates in {elapsed:.2f} s")
+    brick_fused = None if args.no_fusion else brick_fused_measure(sv, layers, torch)
+    if brick_fused:
+        log(f"fused brick circuit: {brick_fused['gates']} gates in {brick_fused['blocks']} blocks, {brick_fused['s']:.2f} s")
 
     # ---- the same circuit undone: distance back to |0...0> (FP64 error of the round trip) ----
     for ops in reversed(layers):
@@ -717,10 +811,11 @@ def main() -> int:
     log("QAOA gradient done")
     fusion = None if args.no_fusion else fusion_measure(args.fusion_qubits, 200, torch)
     log("fusion done")
-    batch = None if args.no_batch else batch_measure(args.batch_reps, torch)
+    batch = None if args.no_batch else batch_measure(args.batch_reps, torch, batch_tune)
     log("HEA batch done")
     cpu = None if args.no_cpu else cpu_baseline(n, layers[0])
     if cpu is not None:
+        cpu["per_kind_30q"] = cpu_kind_sample(n)
         cpu["brick_20q_depth20"] = brick20_against_reference(torch)
         cpu["qaoa_16q_param_shift"] = None if args.no_qaoa else qaoa_param_shift_on_reference(torch)
         cpu["hea_batch_sample"] = None if args.no_batch else hea_sample_on_reference(torch)
@@ -753,6 +848,7 @@ def main() -> int:
                                              len(dense + per_kind.get("dense1q_lane", []))) / 1e9) / HBM_PEAK_GBS,
         "z_readout_first4": readout[:4],
         "brick_roundtrip_max_abs_err": roundtrip,
+        "brick_fused": brick_fused,
         "sweep": sweep,
         "sweep_permute": perm,
         "sweep_reductions": red,

@@ -18,6 +18,11 @@
  *     return host values synchronize that stream; the others are asynchronous.
  *   - Reductions accumulate in FP64 for both precisions, in a fixed two-stage order
  *     (bitwise reproducible run to run).
+ *   - One library stream per process.  Host threads may share the library: the shared
+ *     workspaces (reduction partial sums, block-matrix ring, layer partial sums) are each
+ *     used under a lock held until the call has finished with them, and all work is
+ *     ordered on the one stream.  Switching the stream while another thread has work in
+ *     flight is the caller's to order.
  */
 #ifndef IQS_HIP_H
 #define IQS_HIP_H
@@ -113,8 +118,10 @@ int iqsh_apply_2q(iqsh_handle h, unsigned ph, unsigned pl, const double m[32]);
  * 1 <= k <= 5.  Replaces the reference's fused-gate replay (src/qureg_fusion.cpp:56-94):
  * a run of gates is multiplied into one block on the host and applied in one pass. */
 int iqsh_apply_kq(iqsh_handle h, unsigned k, const unsigned *pos, const double *U);
-/* Select the kernel for 5-qubit blocks: matrix cores (1, default) or vector ALUs (0). */
-int iqsh_set_block_mfma(int enable);
+/* Select the kernel for 5-qubit blocks (and 3-/4-qubit blocks padded to 5): 1 (default)
+ * matrix cores, three real 32x32 products per block (Ur xr, Ui xi, (Ur+Ui)(xr+xi));
+ * 2 matrix cores, one real 64x64 product; 0 vector ALUs.  Process-wide. */
+int iqsh_set_block_mfma(int mode);
 /* state *= f (AmplitudeWiseScalarMultiplication / Normalize, src/qureg_utils.cpp:162-201). */
 int iqsh_scale(iqsh_handle h, double re, double im);
 /* h += f * x (AmplitudeWiseSum, src/qureg_utils.cpp:208-231). */
@@ -160,7 +167,8 @@ int iqsh_qaoa_phase(iqsh_handle h, unsigned ne, const unsigned *u, const unsigne
 int iqsh_qaoa_expect(iqsh_handle h, unsigned ne, const unsigned *u, const unsigned *v,
                      const double *w, double *out);
 /* Write cut_i into the real part of a cost register (InitializeVectorAs[Weighted]MaxCut-
- * CostFunction, :73-239); returns the maximum cut in *max_cut. */
+ * CostFunction, :73-239); returns the maximum cut in *max_cut.  Any number of edges (one
+ * pass per 64 edges); the on-the-fly functions above take at most 64. */
 int iqsh_maxcut_diag(iqsh_handle diag, unsigned ne, const unsigned *u, const unsigned *v,
                      const double *w, double *max_cut);
 /* Reference form with a cost register: psi_i *= (cos(g d_i), -sin(g d_i)), d_i = Re diag_i. */

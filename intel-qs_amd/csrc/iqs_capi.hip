@@ -22,7 +22,7 @@ namespace {
 
 thread_local char g_err[512] = "";
 hipStream_t g_stream = nullptr;
-bool g_block_mfma = true;  // 5-qubit blocks on the matrix cores (iqsh_set_block_mfma)
+int g_block_mfma = 1;  // 5-qubit block kernel (iqsh_set_block_mfma): 0 VALU, 1 MFMA 3 products, 2 MFMA 4 products
 
 int fail(int code, const char* fmt, ...)
 {
@@ -51,6 +51,10 @@ struct RedWork {
 };
 RedWork g_red;
 std::mutex g_mu;
+// One reduction at a time uses g_red: a reduction holds this from its first launch until
+// red_finish has synchronized the stream, so host threads sharing the library do not race
+// on the partial sums.
+std::mutex g_red_mu;
 
 int red_ws(RedWork** out)
 {
@@ -692,9 +696,11 @@ __global__ __launch_bounds__(kBlock) void k_qaoa_diag_expect(const typename Vec<
   block_sum_store<1>(v, partial);
 }
 
+// diag_i := cut_i over the edges of cs (accumulate: diag_i += that), and the maximum of
+// the values written, per workgroup.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_maxcut_fill(typename Vec<T>::t* __restrict__ diag, uint64_t N, CutSpec cs,
-                                                        double* partial)
+                                                        int accumulate, double* partial)
 {
   __shared__ uint64_t tab[8][256];
   __shared__ double wt[8][256];
@@ -706,10 +712,10 @@ __global__ __launch_bounds__(kBlock) void k_maxcut_fill(typename Vec<T>::t* __re
     int ic;
     const double c = cut_value(tab, wt, cs, i, &ic);
     typename Vec<T>::t d;
-    d.x = (T)c;
+    d.x = accumulate ? (T)(c + (double)diag[i].x) : (T)c;
     d.y = 0;
     diag[i] = d;
-    m = fmax(m, c);
+    m = fmax(m, (double)d.x);
   }
   block_max_store(m, partial);
 }
@@ -827,9 +833,10 @@ int iqsh_set_stream(void* stream)
   return IQSH_OK;
 }
 
-int iqsh_set_block_mfma(int enable)
+int iqsh_set_block_mfma(int mode)
 {
-  g_block_mfma = enable != 0;
+  if (mode < 0 || mode > 2) return fail(IQSH_ERR_ARG, "block kernel mode %d (0, 1 or 2)", mode);
+  g_block_mfma = mode;
   return IQSH_OK;
 }
 
@@ -1241,12 +1248,16 @@ int do_block_k(iqsh_handle h, const unsigned* pos, const double* U)
     const uint64_t waves = (tiles + kTilesPerWave - 1) / kTilesPerWave;
     Ins<5> ins5;
     for (int j = 0; j < 5; ++j) ins5.p[j] = ins.p[j < KQ ? j : 0];
-    // complex128: the matrix in LDS and the next tile prefetched (128 -> 160 VGPRs instead of
-    // 224; 30 qubits: +1-5% over the register-resident matrix, identical results); complex64
-    // keeps the matrix in registers (136 VGPRs; the LDS form is no faster there).
     const dim3 grid = grid2((waves + 3) / 4);
     const bool nt = ins5.p[0] >= line_bits<T>();
-    if (sizeof(T) == 8 && nt)
+    // Default: three real products, the matrices in LDS.  Mode 2, the 64 x 64 real form:
+    // complex128 with the matrix in LDS and the next tile prefetched (160 VGPRs instead of
+    // 224), complex64 with the matrix in registers (136 VGPRs).
+    if (g_block_mfma == 1 && nt)
+      k_block5_mfma3<T, kTilesPerWave, true><<<grid, kBlock, 0, g_stream>>>((V*)h->d, tiles, ins5, (const V*)dU);
+    else if (g_block_mfma == 1)
+      k_block5_mfma3<T, kTilesPerWave, false><<<grid, kBlock, 0, g_stream>>>((V*)h->d, tiles, ins5, (const V*)dU);
+    else if (sizeof(T) == 8 && nt)
       k_block5_mfma_lds<T, kTilesPerWave, true, true><<<grid, kBlock, 0, g_stream>>>((V*)h->d, tiles, ins5, (const V*)dU);
     else if (sizeof(T) == 8)
       k_block5_mfma_lds<T, kTilesPerWave, false, true><<<grid, kBlock, 0, g_stream>>>((V*)h->d, tiles, ins5, (const V*)dU);
@@ -1440,6 +1451,7 @@ template <typename T>
 int do_norm_sub(iqsh_handle h, int K, unsigned pos, double* out)
 {
   RedWork* w;
+  std::lock_guard<std::mutex> red_lk(g_red_mu);  // held until red_finish has synchronized
   if (int rc = red_ws(&w)) return rc;
   using V = typename Vec<T>::t;
   if (K == 0) {
@@ -1460,6 +1472,7 @@ template <typename T>
 int do_pauli(iqsh_handle h, uint64_t x, uint64_t z, unsigned ny, double* out2)
 {
   RedWork* w;
+  std::lock_guard<std::mutex> red_lk(g_red_mu);  // held until red_finish has synchronized
   if (int rc = red_ws(&w)) return rc;
   using V = typename Vec<T>::t;
   const unsigned g = red_grid(x ? h->size() / 2 : h->size());
@@ -1472,6 +1485,7 @@ template <typename T>
 int do_two(iqsh_handle h, iqsh_handle x, int kind, double sr, double si, double* out)
 {
   RedWork* w;
+  std::lock_guard<std::mutex> red_lk(g_red_mu);  // held until red_finish has synchronized
   if (int rc = red_ws(&w)) return rc;
   using V = typename Vec<T>::t;
   const unsigned g = red_grid(h->size());
@@ -1503,6 +1517,7 @@ template <typename T>
 int do_maxnorm_half(iqsh_handle h, unsigned pos, int value, double* out)
 {
   RedWork* w;
+  std::lock_guard<std::mutex> red_lk(g_red_mu);  // held until red_finish has synchronized
   if (int rc = red_ws(&w)) return rc;
   Ins<1> ins;
   ins.p[0] = pos;
@@ -1589,6 +1604,7 @@ template <typename T>
 int do_qaoa_cost(iqsh_handle h, iqsh_handle lam, const CutSpec& cs, double* out)
 {
   RedWork* w;
+  std::lock_guard<std::mutex> red_lk(g_red_mu);  // held until red_finish has synchronized
   if (int rc = red_ws(&w)) return rc;
   const unsigned g = red_grid(h->size());
   k_qaoa_cost<T><<<g, kBlock, 0, g_stream>>>((const typename Vec<T>::t*)h->d,
@@ -1602,6 +1618,7 @@ template <typename T>
 int do_qaoa_phase_adj(iqsh_handle h, iqsh_handle lam, const CutSpec& cs, double* out)
 {
   RedWork* w;
+  std::lock_guard<std::mutex> red_lk(g_red_mu);  // held until red_finish has synchronized
   if (int rc = red_ws(&w)) return rc;
   const unsigned g = red_grid(h->size());
   k_qaoa_phase_adj<T><<<g, kBlock, 0, g_stream>>>((typename Vec<T>::t*)h->d, (typename Vec<T>::t*)lam->d, h->size(), cs,
@@ -1614,12 +1631,13 @@ int do_qaoa_phase_adj(iqsh_handle h, iqsh_handle lam, const CutSpec& cs, double*
 }
 
 template <typename T>
-int do_maxcut_fill(iqsh_handle h, const CutSpec& cs, double* out)
+int do_maxcut_fill(iqsh_handle h, const CutSpec& cs, int accumulate, double* out)
 {
   RedWork* w;
+  std::lock_guard<std::mutex> red_lk(g_red_mu);  // held until red_finish has synchronized
   if (int rc = red_ws(&w)) return rc;
   const unsigned g = red_grid(h->size());
-  k_maxcut_fill<T><<<g, kBlock, 0, g_stream>>>((typename Vec<T>::t*)h->d, h->size(), cs, w->partial);
+  k_maxcut_fill<T><<<g, kBlock, 0, g_stream>>>((typename Vec<T>::t*)h->d, h->size(), cs, accumulate, w->partial);
   CHECK_LAUNCH();
   return red_finish(w, g, 1, 1, out);
 }
@@ -1638,6 +1656,7 @@ template <typename T>
 int do_qaoa_diag_expect(iqsh_handle psi, iqsh_handle diag, int squared, double* out)
 {
   RedWork* w;
+  std::lock_guard<std::mutex> red_lk(g_red_mu);  // held until red_finish has synchronized
   if (int rc = red_ws(&w)) return rc;
   const unsigned g = red_grid(psi->size());
   k_qaoa_diag_expect<T><<<g, kBlock, 0, g_stream>>>((const typename Vec<T>::t*)psi->d,
@@ -1650,6 +1669,7 @@ template <typename T>
 int do_adj_1q(iqsh_handle h, iqsh_handle lam, unsigned pos, const double* uinv, const double* G, double* out)
 {
   RedWork* w;
+  std::lock_guard<std::mutex> red_lk(g_red_mu);  // held until red_finish has synchronized
   if (int rc = red_ws(&w)) return rc;
   const uint64_t items = h->size() / 2;
   const unsigned g = red_grid(items);
@@ -1709,10 +1729,17 @@ int iqsh_maxcut_diag(iqsh_handle diag, unsigned ne, const unsigned* u, const uns
 {
   if (int rc = check_h(diag)) return rc;
   if (ne && (!u || !v)) return fail(IQSH_ERR_ARG, "null edge list");
-  CutSpec cs;
-  if (int rc = build_cut(diag, ne, u, v, w, 0.0, &cs)) return rc;
-  double m;
-  if (int rc = DISPATCH(diag, do_maxcut_fill)(diag, cs, &m)) return rc;
+  // Any number of edges: one pass per 64 edges, the later passes adding to the register
+  // (the maximum is that of the last pass, over the complete cuts).
+  double m = 0;
+  unsigned e0 = 0;
+  do {
+    const unsigned k = std::min(ne - e0, 64u);
+    CutSpec cs;
+    if (int rc = build_cut(diag, k, u + e0, v + e0, w ? w + e0 : nullptr, 0.0, &cs)) return rc;
+    if (int rc = DISPATCH(diag, do_maxcut_fill)(diag, cs, e0 > 0, &m)) return rc;
+    e0 += k;
+  } while (e0 < ne);
   if (max_cut) *max_cut = m;
   return IQSH_OK;
 }

@@ -719,6 +719,96 @@ __global__ __launch_bounds__(kBlock) void k_block5_mfma_lds(typename Vec<T>::t* 
   }
 }
 
+// The same block product with three real 32 x 32 products instead of one 64 x 64 one
+// (Gauss's three-multiplication form of a complex product):
+//   T1 = Ur xr,  T2 = Ui xi,  T3 = (Ur + Ui)(xr + xi);   yr = T1 - T2,  yi = T3 - T1 - T2,
+// 48 MFMAs per tile instead of 64 (192 flop per amplitude instead of 256).  The three
+// matrices live in LDS in fragment order, sA[((m * 2 + mb) * 8 + ks) * 64 + lane] =
+// A_m[16 mb + lane%16][4 ks + lane/16] for A_0 = Ur, A_1 = Ui, A_2 = Ur + Ui; B is row
+// c + 4 ks of xr, xi or xr + xi.  Row offsets and accumulator layouts as k_block5_mfma.
+// Rounding: yi carries the error of the larger (Ur + Ui)(xr + xi) product, a few units of
+// the last place of |U||x| per element instead of one.
+template <typename T, int TPW, bool NT>
+__global__ __launch_bounds__(kBlock) void k_block5_mfma3(typename Vec<T>::t* __restrict__ st, uint64_t ntiles,
+                                                         Ins<5> ins, const typename Vec<T>::t* __restrict__ U)
+{
+  using V = typename Vec<T>::t;
+  using Acc = typename MfmaAcc<T>::t;
+  __shared__ T sA[3 * 2 * 8 * 64];
+  for (int e = threadIdx.x; e < 3 * 2 * 8 * 64; e += kBlock) {
+    const int m = e >> 10, mb = (e >> 9) & 1, ks = (e >> 6) & 7, l = e & 63;
+    const V u = U[(16 * mb + (l & 15)) * 32 + 4 * ks + (l >> 4)];
+    sA[e] = m == 0 ? u.x : (m == 1 ? u.y : u.x + u.y);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, c = lane >> 4, gl = lane & 15;
+  const uint64_t G = grid_blocks();
+  const uint64_t bb = (G & 7) ? block_id() : xcd_block(block_id(), G);
+  const uint64_t wave = bb * (kBlock / 64) + (threadIdx.x >> 6);
+  const uint64_t t0 = wave * TPW;
+  if (t0 >= ntiles) return;
+  uint64_t offin[8], offout[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int rin = c + 4 * j;
+    const int i = j & 3;
+    const int rin4 = (sizeof(T) == 8) ? (c + 4 * i) : (4 * c + i);
+    const int rout = (j < 4) ? rin4 : (16 + rin4);
+    uint64_t oi = 0, oo = 0;
+#pragma unroll
+    for (int b = 0; b < 5; ++b) {
+      if ((rin >> b) & 1) oi |= uint64_t(1) << ins.p[b];
+      if ((rout >> b) & 1) oo |= uint64_t(1) << ins.p[b];
+    }
+    offin[j] = oi;
+    offout[j] = oo;
+  }
+  const uint64_t t1 = (t0 + TPW < ntiles) ? t0 + TPW : ntiles;
+  uint64_t base = item_base(t0 * 16 + gl, ins);
+  V x[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) x[j] = ldx<NT>(&st[base | offin[j]]);
+  for (uint64_t t = t0; t < t1; ++t) {
+    // Keep the A reads inside the loop: hoisted out of it, A would occupy registers.
+    __asm__ volatile("" ::: "memory");
+    const bool more = t + 1 < t1;
+    const uint64_t nbase = more ? item_base((t + 1) * 16 + gl, ins) : base;
+    V xn[8];
+    if (more) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) xn[j] = ldx<NT>(&st[nbase | offin[j]]);
+    }
+    Acc d1[2], d2[2], d3[2];
+#pragma unroll
+    for (int mb = 0; mb < 2; ++mb) d1[mb] = d2[mb] = d3[mb] = Acc{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+      const T br = x[ks].x, bi = x[ks].y, bs = br + bi;
+#pragma unroll
+      for (int mb = 0; mb < 2; ++mb) {
+        d1[mb] = mfma16x16x4(sA[((0 + mb) * 8 + ks) * 64 + lane], br, d1[mb]);
+        d2[mb] = mfma16x16x4(sA[((2 + mb) * 8 + ks) * 64 + lane], bi, d2[mb]);
+        d3[mb] = mfma16x16x4(sA[((4 + mb) * 8 + ks) * 64 + lane], bs, d3[mb]);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      V y0, y1;
+      y0.x = d1[0][i] - d2[0][i];
+      y0.y = d3[0][i] - d1[0][i] - d2[0][i];
+      y1.x = d1[1][i] - d2[1][i];
+      y1.y = d3[1][i] - d1[1][i] - d2[1][i];
+      stx<NT>(y0, &st[base | offout[i]]);
+      stx<NT>(y1, &st[base | offout[4 + i]]);
+    }
+    if (more) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) x[j] = xn[j];
+    }
+    base = nbase;
+  }
+}
+
 // ---------------------------------------------------------------------------------------
 // Fills.
 template <typename T>

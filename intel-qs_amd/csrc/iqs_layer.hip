@@ -170,8 +170,14 @@ __global__ __launch_bounds__(kLB) void k_layer_low(typename Vec<T>::t* __restric
 // before the rotations; adjoint (run as the last pass of the backward step, after every
 // other rotation of the layer is undone), the rotation terms, the undo, then the phase term
 // cut_i Im(conj(lam_i) psi_i) and psi, lam *= exp(+i gamma cut) (cs built with -gamma).
-// part receives two sums per workgroup (rotations, phase).  W: weighted cuts (byte tables
-// of weight sums in LDS); unit weights only need the incidence tables and the phase table.
+// part receives two sums per workgroup (rotations, phase).
+//
+// The cut edges of index x are E(x) = XOR of inc[b] over the set bits b of x (bit e of
+// inc[b]: edge e touches position b), which is linear over GF(2): E(row | lo) =
+// E(row) ^ E(lo).  Unit weights (cut = popcount E, phase from a 65-entry table): each thread
+// takes E(lo) of its P in-row offsets once, and E(row) comes from the row's set bits, the
+// same for the whole workgroup -- no per-workgroup tables, so short workgroups as in the
+// plain lane pass.  W: weighted cuts, summed from byte tables of weight sums in LDS.
 template <typename T, bool ADJ, bool W>
 __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict__ psi, typename Vec<T>::t* __restrict__ lam,
                                                   uint64_t N, int L, Mat2<T> u, Mat2<double> g, CutSpec cs,
@@ -181,21 +187,39 @@ __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict
   // The adjoint holds psi and lambda: 2 amplitudes of each per lane keep it at 4 waves per
   // SIMD instead of 2.
   constexpr int P = ADJ ? 2 : kLowP;
-  __shared__ uint64_t tab[8][256];
-  __shared__ double wt[W ? 8 : 1][256];
-  __shared__ Cx<T> ph[65];
-  build_tables(tab, cs.inc);
-  if (W) build_weight_tables(wt, cs);
-  for (unsigned c = threadIdx.x; c <= cs.ne && c < 65; c += kLB) ph[c] = Cx<T>{(T)cs.ph_re[c], (T)cs.ph_im[c]};
+  constexpr int RB = ADJ ? 9 : 10;  // log2 of a row's amplitudes
+  static_assert(kLB * P == (1 << RB), "row size");
+  __shared__ uint64_t tab[W ? 8 : 1][W ? 256 : 1];
+  __shared__ double wt[W ? 8 : 1][W ? 256 : 1];
+  __shared__ Cx<T> ph[W ? 1 : 65];
+  uint64_t elo[P];
+  if constexpr (W) {
+    build_tables(tab, cs.inc);
+    build_weight_tables(wt, cs);
+  } else {
+    for (unsigned c = threadIdx.x; c <= cs.ne; c += kLB) ph[c] = Cx<T>{(T)cs.ph_re[c], (T)cs.ph_im[c]};
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      const unsigned xl = threadIdx.x + j * kLB;
+      uint64_t e = 0;
+#pragma unroll
+      for (int b = 0; b < RB; ++b)
+        if ((xl >> b) & 1) e ^= cs.inc.c[b];
+      elo[j] = e;
+    }
+  }
   __syncthreads();
   const int lane = threadIdx.x & 63;
   double acc = 0, accg = 0;
-  const uint64_t rows = N / (kLB * P), G = gridDim.x;
+  const uint64_t rows = N >> RB, G = gridDim.x;
   const uint64_t lb = (G & 7) ? blockIdx.x : xcd_block(blockIdx.x, G);
   const uint64_t chunk = (rows + G - 1) / G;
   const uint64_t b0 = lb * chunk, b1 = (b0 + chunk < rows) ? b0 + chunk : rows;
   for (uint64_t rb = b0; rb < b1; ++rb) {
-    const uint64_t r0 = rb * (kLB * P) + threadIdx.x;
+    const uint64_t r0 = (rb << RB) + threadIdx.x;
+    uint64_t ehi = 0;
+    if constexpr (!W)
+      for (uint64_t r = rb; r; r &= r - 1) ehi ^= cs.inc.c[RB + __builtin_ctzll(r)];
     V x[P], y[P];
     Cx<T> f[P];
     double cut[P];
@@ -203,11 +227,13 @@ __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict
     for (int j = 0; j < P; ++j) {
       x[j] = ldg(&psi[r0 + (uint64_t)j * kLB]);
       if (ADJ) y[j] = ldg(&lam[r0 + (uint64_t)j * kLB]);
-      int ic;
-      cut[j] = cut_value(tab, wt, cs, r0 + (uint64_t)j * kLB, &ic);
-      if (!W) {
+      if constexpr (!W) {
+        const int ic = __popcll(ehi ^ elo[j]);
+        cut[j] = (double)ic;
         f[j] = ph[ic];
       } else {
+        int ic;
+        cut[j] = cut_value(tab, wt, cs, r0 + (uint64_t)j * kLB, &ic);
         double sn, co;
         sincos(cs.gamma * cut[j], &sn, &co);
         f[j] = Cx<T>{(T)co, (T)-sn};
@@ -405,11 +431,11 @@ int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm,
   // Forward: the lane pass first (it carries the phase layer, applied before the mixer);
   // adjoint: last (the phase is undone after the whole mixer).
   auto low_pass = [&]() -> int {
-    // The QAOA form builds its cut tables per workgroup: a persistent grid of <= kMaxGrid
-    // workgroups (28 qubits: 1.87 ms forward against 2.08 ms with kLowRows rows each).
-    const uint64_t rows = h->size() / (kLB * kLowP);
-    const unsigned grid = cs ? (unsigned)std::min<uint64_t>(rows, kMaxGrid)
-                             : (unsigned)std::min<uint64_t>(std::max<uint64_t>(1, rows / kLowRows), kLowMaxGrid);
+    // kLowRows rows per workgroup.  The weighted QAOA form builds byte tables of weight sums
+    // per workgroup: a persistent grid of <= kMaxGrid workgroups for it.
+    const uint64_t rows = h->size() / (kLB * ((ADJ && cs) ? 2 : kLowP));
+    const unsigned grid = (cs && cs->weighted) ? (unsigned)std::min<uint64_t>(rows, kMaxGrid)
+                                               : (unsigned)std::min<uint64_t>(std::max<uint64_t>(1, rows / kLowRows), kLowMaxGrid);
     if (ADJ)
       if (int rc = part_buf(2 * (uint64_t)grid + 2)) return rc;
     V* lp = ADJ ? (V*)lam->d : nullptr;

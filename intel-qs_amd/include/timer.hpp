@@ -1,8 +1,11 @@
 /// @file timer.hpp
 /// Per-gate statistics of the drop-in (the reference's Timer, include/timer.hpp:129-341,
 /// records host time and a bandwidth model per gate).  Here each Start/Stop pair
-/// synchronizes the device and records wall time and the algorithmic bytes of the gate, so
-/// Breakdown() prints achieved GB/s per gate name.  Enabled by EnableStatistics().
+/// synchronizes the device and records wall time and the algorithmic bytes of the gate.
+/// Breakdown() prints what the reference's single-process build prints (its per-gate table
+/// needs MPI, include/timer.hpp:320-323), so programs print the same text through the
+/// drop-in; BreakdownRates() prints the achieved GB/s per gate name.  Enabled by
+/// EnableStatistics().
 #pragma once
 
 #include <cstdio>
@@ -31,6 +34,11 @@ class Timer
   }
   void Reset() { recs_.clear(); }
   void Breakdown() const
+  {
+    printf(" *** The statistics (i.e. time used in computation and bandwidth) are available only when MPI is used inside Intel QS.\n");
+    printf(" *** This is not the case for this simulation. If needed, rebuild the IQS adding the CMake option '-DIqsMPI=ON'.\n");
+  }
+  void BreakdownRates() const
   {
     printf("%-40s %8s %12s %10s\n", "gate", "calls", "time[s]", "GB/s");
     for (auto const& kv : recs_)

@@ -53,7 +53,7 @@ def test_fixture_through_dropin(name):
 
 
 @pytest.mark.parametrize("name", ["brick10_dp", "brick10_sp", "random1q2q_dp", "random1q2q_sp", "gates2q_dp"])
-@pytest.mark.parametrize("k", [2, 3, 5])
+@pytest.mark.parametrize("k", [1, 2, 3, 5])
 def test_fusion_matches_unfused_reference(name, k):
     c, _, g = load(name)
     reg = runner.new_register(c)
@@ -202,6 +202,36 @@ def test_qaoa_ring_and_weighted_cuts():
     iqs.InitializeVectorAsWeightedMaxCutCostFunction(dw, w)
     assert abs(dw.GetGlobalAmplitude(4) - 2.4) < 1e-15
     assert abs(dw.GetGlobalAmplitude(36) - 4.8) < 1e-15
+
+
+@pytest.mark.parametrize("cls", ["QubitRegister", "QubitRegisterSP"])
+def test_cost_register_of_dense_graphs(cls):
+    """More edges than one 64-bit edge mask: the complete graph K_13 (78 edges) has cut
+    k (13 - k) for a basis index with k ones (max 42), and a weighted dense graph (random
+    weights, 70 edges) against the brute-force cut of every index."""
+    n = 13
+    reg = getattr(iqs, cls)
+    diag = reg(n, "base", 0, 0)
+    adj = [0 if u == v else 1 for v in range(n) for u in range(n)]
+    assert iqs.InitializeVectorAsMaxCutCostFunction(diag, adj) == 42
+    vals = np.array(diag, copy=True).real
+    k = np.array([bin(i).count("1") for i in range(1 << n)])
+    assert np.array_equal(vals, (k * (n - k)).astype(vals.dtype))
+    r = np.random.default_rng(13)
+    w = np.zeros((n, n))
+    edges = [(a, b) for a in range(n) for b in range(a + 1, n)]
+    for e in r.permutation(len(edges))[:70]:
+        a, b = edges[e]
+        w[a, b] = w[b, a] = round(float(r.uniform(0.5, 2.0)), 3)
+    dw = reg(n, "base", 0, 0)
+    mx = iqs.InitializeVectorAsWeightedMaxCutCostFunction(dw, [float(x) for x in w.ravel()])
+    idx = np.arange(1 << n)
+    bits = (idx[:, None] >> np.arange(n)) & 1
+    want = np.einsum("ia,ab,ib->i", bits, w, 1 - bits)  # sum of w over edges with different bits
+    got = np.array(dw, copy=True).real
+    tol = 1e-12 if cls == "QubitRegister" else 1e-4
+    assert np.abs(got - want).max() < tol
+    assert abs(mx - want.max()) < tol
 
 
 def test_gate_counter_and_toffoli_truth_table():

@@ -251,7 +251,7 @@ def test_fused_blocks_32q_against_unfused_fp64():
         reg.TurnOnFusion()
         runner.run(c, reg)
         gates, blocks = reg.FusionStats()
-        assert gates == 200 and blocks < 200
+        assert gates == 200 and blocks <= 26  # dependency-graph planner (59 when closed at the first misfit)
         a = h(reg)
         tmp = None
         if prec == "sp":

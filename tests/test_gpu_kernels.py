@@ -434,11 +434,15 @@ def test_block5_mfma_vs_valu_and_gather(dt, positions):
     U = haar_unitary(32, SplitMix64(sum(p * (i + 3) for i, p in enumerate(positions))))
     h = rand_state(n, dt, 41)
     sv = on_gpu(h, dt)
-    sv.apply_kq(positions, U)  # matrix cores
+    sv.apply_kq(positions, U)  # matrix cores, three real products
     got = sv.download()
     sv.upload(h)
-    call("iqsh_set_block_mfma", 0)
+    call("iqsh_set_block_mfma", 2)
     try:
+        sv.apply_kq(positions, U)  # matrix cores, one 64 x 64 real product
+        got4 = sv.download()
+        sv.upload(h)
+        call("iqsh_set_block_mfma", 0)
         sv.apply_kq(positions, U)  # vector ALUs
     finally:
         call("iqsh_set_block_mfma", 1)
@@ -452,7 +456,15 @@ def test_block5_mfma_vs_valu_and_gather(dt, positions):
     want[idx] = U @ a[idx]
     t = 1e-12 if dt == "c128" else 3e-5
     assert np.abs(got - want).max() < t
+    assert np.abs(got4 - want).max() < t
     assert np.abs(valu - want).max() < t
+
+
+def test_block_kernel_mode_is_checked():
+    from iqs_amd._lib import IqsError, call
+    for bad in (-1, 3):
+        with pytest.raises(IqsError):
+            call("iqsh_set_block_mfma", bad)
 
 
 @pytest.mark.parametrize("positions", [[0, 1, 2], [13, 2, 7], [12, 13, 11], [1, 3, 5, 7], [12, 13, 0, 6], [3, 2, 1, 0]])

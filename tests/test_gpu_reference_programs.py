@@ -5,7 +5,8 @@ Both builds come from oracle/Makefile (target `progs`, made by build() where the
 sources exist): oracle/_ref/progs/ref/<name> links the reference compiled from its sources,
 oracle/_ref/progs/dropin/<name> the same source against intel-qs_amd/include and
 lib/libiqs_amd.so.  Output is compared line by line: the text between numbers must be
-identical, and numbers must agree to 1e-6 (absolute, plus 1e-6 relative).  Numbers on lines
+identical up to the width of the runs of blanks (a printed -0.00000000 is one column wider
+than 0.00000000), and numbers must agree to 1e-6 (absolute, plus 1e-6 relative).  Numbers on lines
 that report an error against another method (the QFT-vs-FFT lines) only have to be below
 1e-5 in both runs, since they measure round-off.  Timing and thread-count lines are
 dropped.  References: examples/*.cpp, tutorials/*.cpp, benchmarks/specv2_bench.cpp.
@@ -48,7 +49,7 @@ def outputs_agree(got, want):
     if len(got) != len(want):
         return f"{len(got)} lines vs {len(want)}"
     for k, (g, w) in enumerate(zip(got, want)):
-        if NUM.sub("#", g) != NUM.sub("#", w):
+        if NUM.sub("#", g).split() != NUM.sub("#", w).split():
             return f"line {k}: {g!r} vs {w!r}"
         err_line = re.search(r"error|diff", w, re.IGNORECASE)
         for a, b in zip(NUM.findall(g), NUM.findall(w)):
@@ -70,6 +71,8 @@ def test_comparator_catches_differences():
     assert outputs_agree(["error max(absdiff: 3e-08)"], ["error max(absdiff: 7e-08)"]) is None
     assert outputs_agree(["error max(absdiff: 3e-03)"], ["error max(absdiff: 7e-08)"]) is not None
     assert outputs_agree(a, a[:1]) is not None
+    assert outputs_agree(["\t0.7    + i * -0.00000000     % |1>"], ["\t0.7    + i * 0.00000000      % |1>"]) is None
+    assert outputs_agree(["\t0.7    + i * 0.0     % |1>"], ["\t0.7    + i * 0.0     % |0>"]) is not None
 
 
 @pytest.mark.gpu

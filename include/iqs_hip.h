@@ -99,6 +99,19 @@ int iqsh_init_uniform(iqsh_handle h, double re, double im);
 int iqsh_apply_1q(iqsh_handle h, unsigned pos, const double m[8]);
 /* SpecializeV2 1q bodies (src/spec_kernels.cpp:36-102, Loop_DN spec :249-332). */
 int iqsh_apply_1q_spec(iqsh_handle h, unsigned pos, int spec, double angle);
+/* The 2x2 m on the pairs (k, k + c[8]) of the reference's Loop_TN loop nest
+ * (src/highperfkernels.cpp:397-486): for (i = c[0]; i < c[1]; i += c[2])
+ * for (j = i + c[3]; j < i + c[4]; j += c[5]) for (k = j + c[6]; k < j + c[7]; ++k).
+ * Loop_DN(gstart, gend, pos) is {gstart, gend, 2^(pos+1), 0, 1, 1, 0, 2^pos, 2^pos}.  For the
+ * sub-range gate helpers (Apply1QubitGate_helper / ApplyControlled1QubitGate_helper with
+ * sind, eind: src/qureg_apply1qubitgate.cpp:173-230, src/qureg_applyctrl1qubitgate.cpp:
+ * 228-406).  Every index touched must lie in the register (checked); the pairs must be
+ * disjoint, as in the reference. */
+int iqsh_apply_nest(iqsh_handle h, const uint64_t c[9], const double m[8]);
+/* The 2x2 matrix the SpecializeV2 path applies for `spec` (controlled: GateSpec2Q meaning
+ * of IQSH_SPEC_T_OR_CPHASE), with the reference's constants in `dtype`'s precision
+ * (src/spec_kernels.cpp:172-178, 264-270, 356-361). */
+int iqsh_spec_matrix(int dtype, int spec, int controlled, double angle, double m[8]);
 /* Controlled 2x2 on target tpos where bit cpos = 1: Loop_TN (src/highperfkernels.cpp:397-486)
  * as called by ApplyControlled1QubitGate_helper (src/qureg_applyctrl1qubitgate.cpp:228-406). */
 int iqsh_apply_c1q(iqsh_handle h, unsigned cpos, unsigned tpos, const double m[8]);

@@ -711,11 +711,12 @@ __global__ __launch_bounds__(kBlock) void k_maxcut_fill(typename Vec<T>::t* __re
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < N; i += (uint64_t)gridDim.x * kBlock) {
     int ic;
     const double c = cut_value(tab, wt, cs, i, &ic);
+    const double tot = accumulate ? c + (double)diag[i].x : c;
     typename Vec<T>::t d;
-    d.x = accumulate ? (T)(c + (double)diag[i].x) : (T)c;
+    d.x = (T)tot;
     d.y = 0;
     diag[i] = d;
-    m = fmax(m, (double)d.x);
+    m = fmax(m, tot);
   }
   block_max_store(m, partial);
 }
@@ -1014,20 +1015,68 @@ int iqsh_apply_1q(iqsh_handle h, unsigned pos, const double m[8])
   return DISPATCH(h, controlled_2x2)(h, 0, pos, m);
 }
 
+int iqsh_spec_matrix(int dtype, int spec, int controlled, double angle, double m[8])
+{
+  if (dtype != IQSH_C64 && dtype != IQSH_C128) return fail(IQSH_ERR_ARG, "bad dtype %d", dtype);
+  if (!m) return fail(IQSH_ERR_ARG, "null matrix");
+  if (spec == IQSH_SPEC_T_OR_CPHASE && !controlled) {
+    const double mt[8] = {1, 0, 0, 0, 0, 0, std::cos(M_PI / 4), std::sin(M_PI / 4)};
+    std::memcpy(m, mt, 8 * sizeof(double));
+  } else if (spec == IQSH_SPEC_T_OR_CPHASE) {
+    // CPhase: pexp = (cos(theta), sin(theta)) in the register precision.
+    if (dtype == IQSH_C128) {
+      const double mp[8] = {1, 0, 0, 0, 0, 0, std::cos(angle), std::sin(angle)};
+      std::memcpy(m, mp, 8 * sizeof(double));
+    } else {
+      const float th = (float)angle;
+      const double mp[8] = {1, 0, 0, 0, 0, 0, (double)std::cos(th), (double)std::sin(th)};
+      std::memcpy(m, mp, 8 * sizeof(double));
+    }
+  } else if (spec >= IQSH_SPEC_H && spec <= IQSH_SPEC_Z) {
+    if (dtype == IQSH_C128) spec_matrix<double>(spec, angle, m);
+    else spec_matrix<float>(spec, angle, m);
+  } else
+    return fail(IQSH_ERR_ARG, "bad spec %d", spec);
+  return IQSH_OK;
+}
+
 int iqsh_apply_1q_spec(iqsh_handle h, unsigned pos, int spec, double angle)
 {
   if (int rc = check_h(h)) return rc;
   if (int rc = check_pos(h, pos)) return rc;
   double m[8];
-  if (spec == IQSH_SPEC_T_OR_CPHASE) {
-    const double mt[8] = {1, 0, 0, 0, 0, 0, std::cos(M_PI / 4), std::sin(M_PI / 4)};
-    std::memcpy(m, mt, sizeof m);
-  } else if (spec >= IQSH_SPEC_H && spec <= IQSH_SPEC_Z) {
-    if (h->dtype == IQSH_C128) spec_matrix<double>(spec, angle, m);
-    else spec_matrix<float>(spec, angle, m);
-  } else
-    return fail(IQSH_ERR_ARG, "bad spec %d", spec);
+  if (int rc = iqsh_spec_matrix(h->dtype, spec, 0, angle, m)) return rc;
   return DISPATCH(h, controlled_2x2)(h, 0, pos, m);
+}
+
+namespace {
+template <typename T>
+int do_nest(iqsh_handle h, const Nest& nt, uint64_t items, const double* m)
+{
+  const Mat2<T> mm{{(T)m[0], (T)m[1]}, {(T)m[2], (T)m[3]}, {(T)m[4], (T)m[5]}, {(T)m[6], (T)m[7]}};
+  k_nest<T><<<grid2(std::min<uint64_t>(grid_for(items, kBlock), uint64_t(1) << 20)), kBlock, 0, g_stream>>>(
+      (typename Vec<T>::t*)h->d, nt, items, mm);
+  CHECK_LAUNCH();
+  return IQSH_OK;
+}
+}  // namespace
+
+int iqsh_apply_nest(iqsh_handle h, const uint64_t c[9], const double m[8])
+{
+  if (int rc = check_h(h)) return rc;
+  if (!c || !m) return fail(IQSH_ERR_ARG, "null argument");
+  const uint64_t c11 = c[0], c12 = c[1], c13 = c[2], c21 = c[3], c22 = c[4], c23 = c[5], c31 = c[6], c32 = c[7],
+                 shift = c[8];
+  if (!c13 || !c23 || !shift) return fail(IQSH_ERR_ARG, "zero stride or shift");
+  if (c12 <= c11 || c22 <= c21 || c32 <= c31) return IQSH_OK;  // empty nest: nothing to do
+  Nest nt{c11, c13, c21, c23, c31, (c12 - c11 + c13 - 1) / c13, (c22 - c21 + c23 - 1) / c23, c32 - c31, shift};
+  // The last index touched must lie in the register; overflow-checked in long double.
+  const long double last = (long double)c11 + (long double)(nt.ni - 1) * c13 + c21 + (long double)(nt.nj - 1) * c23 +
+                           c32 - 1 + shift;
+  if (last >= (long double)h->size()) return fail(IQSH_ERR_ARG, "loop nest reaches index %.0Lf of %llu", last,
+                                                   (unsigned long long)h->size());
+  if ((long double)nt.ni * nt.nj * nt.nk > 1.9e19L) return fail(IQSH_ERR_ARG, "loop nest too large");
+  return DISPATCH(h, do_nest)(h, nt, nt.ni * nt.nj * nt.nk, m);
 }
 
 int iqsh_apply_c1q(iqsh_handle h, unsigned c, unsigned t, const double m[8])
@@ -1047,21 +1096,7 @@ int iqsh_apply_c1q_spec(iqsh_handle h, unsigned c, unsigned t, int spec, double 
   if (int rc = check_pos(h, t)) return rc;
   if (c == t) return fail(IQSH_ERR_ARG, "control and target coincide");
   double m[8];
-  if (spec == IQSH_SPEC_T_OR_CPHASE) {
-    // CPhase: pexp = (cos(theta), sin(theta)) in the register precision.
-    if (h->dtype == IQSH_C128) {
-      const double mp[8] = {1, 0, 0, 0, 0, 0, std::cos(angle), std::sin(angle)};
-      std::memcpy(m, mp, sizeof m);
-    } else {
-      const float th = (float)angle;
-      const double mp[8] = {1, 0, 0, 0, 0, 0, (double)std::cos(th), (double)std::sin(th)};
-      std::memcpy(m, mp, sizeof m);
-    }
-  } else if (spec >= IQSH_SPEC_H && spec <= IQSH_SPEC_Z) {
-    if (h->dtype == IQSH_C128) spec_matrix<double>(spec, angle, m);
-    else spec_matrix<float>(spec, angle, m);
-  } else
-    return fail(IQSH_ERR_ARG, "bad spec %d", spec);
+  if (int rc = iqsh_spec_matrix(h->dtype, spec, 1, angle, m)) return rc;
   return DISPATCH(h, controlled_2x2)(h, uint64_t(1) << c, t, m);
 }
 
@@ -1376,19 +1411,21 @@ int do_permute(iqsh_handle h, const unsigned* src_pos)
       return fail(IQSH_ERR_ALLOC, "permutation buffer: %s", hipGetErrorString(e));
     }
   }
-  // Tile bits: the kPermLow lowest destination bits and every destination bit whose source
-  // position is below kPermLow.  If that adds bits (low positions move up), the tiled
-  // transpose reads and writes whole runs; otherwise the gather's reads are already runs.
+  // Tile bits: the L = perm_low<T>() lowest destination bits (512-byte runs) and every
+  // destination bit whose source position is below L.  If that adds bits (low positions
+  // move up), the tiled transpose reads and writes whole runs; otherwise the gather's reads
+  // are already runs.
+  constexpr unsigned L = perm_low<T>();
   const unsigned n = h->n;
   bool in_tile[64] = {};
-  for (unsigned b = 0; b < n; ++b) in_tile[b] = b < kPermLow || src_pos[b] < kPermLow;
+  for (unsigned b = 0; b < n; ++b) in_tile[b] = b < L || src_pos[b] < L;
   unsigned m = 0;
   for (unsigned b = 0; b < n; ++b) m += in_tile[b];
-  const bool tiled = n >= 16 && m > kPermLow;
-  // Pad the tile to 2 kPermLow bits with the lowest remaining destination bits, so every
-  // workgroup moves 2^10 amplitudes (a 6-bit tile of 64 left 3/4 of the threads idle and
-  // ran at 0.85 TB/s).
-  for (unsigned b = 0; tiled && b < n && m < 2 * kPermLow; ++b)
+  const bool tiled = n >= 16 && m > L;
+  // Pad the tile to 2L bits with the lowest remaining destination bits, so every workgroup
+  // moves 2^(2L) amplitudes (a 6-bit complex128 tile of 64 left 3/4 of the threads idle
+  // and ran at 0.85 TB/s).
+  for (unsigned b = 0; tiled && b < n && m < 2 * L; ++b)
     if (!in_tile[b]) {
       in_tile[b] = true;
       ++m;

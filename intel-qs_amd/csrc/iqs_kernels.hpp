@@ -811,6 +811,27 @@ __global__ __launch_bounds__(kBlock) void k_block5_mfma3(typename Vec<T>::t* __r
 
 // ---------------------------------------------------------------------------------------
 // Fills.
+// Loop nest of the reference's Loop_TN (src/highperfkernels.cpp:397-486): for i = c11 +
+// ii c13, j = i + c21 + jj c23, k = j + c31 + kk (ii < ni, jj < nj, kk < nk), the 2x2 m on
+// the pair (k, k + shift).  The reference's sub-range gate helpers reduce to it (Loop_DN is
+// the nest with c21 = 0, nj = 1).  A rare path (ranges of a cache-blocked caller): one item
+// per pair with 64-bit divisions, grid-stride.
+struct Nest {
+  uint64_t c11, c13, c21, c23, c31, ni, nj, nk, shift;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_nest(typename Vec<T>::t* __restrict__ st, Nest nt, uint64_t items, Mat2<T> m)
+{
+  for (uint64_t w = block_id() * kBlock + threadIdx.x; w < items; w += grid_blocks() * kBlock) {
+    const uint64_t kk = w % nt.nk, r = w / nt.nk, jj = r % nt.nj, ii = r / nt.nj;
+    const uint64_t i0 = nt.c11 + ii * nt.c13 + nt.c21 + jj * nt.c23 + nt.c31 + kk, i1 = i0 + nt.shift;
+    const typename Vec<T>::t a = st[i0], b = st[i1];
+    st[i0] = cmad2(m.m00, a, m.m01, b);
+    st[i1] = cmad2(m.m10, a, m.m11, b);
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fill(typename Vec<T>::t* __restrict__ st, uint64_t N, Cx<T> v)
 {
@@ -893,11 +914,14 @@ __global__ __launch_bounds__(kBlock) void k_permute(typename Vec<T>::t* __restri
 
 // Tiled bit permutation, for permutations that move low positions up (where the gather
 // above reads with a stride).  A tile is the set of 2^m destination indices that differ only
-// in the tile bits: the kPermLow lowest destination bits plus every destination bit whose
-// source position is below kPermLow (m <= 2 kPermLow).  The tile is read in source order
+// in the tile bits: the perm_low<T>() lowest destination bits plus every destination bit
+// whose source position is below that (m <= 2 perm_low<T>()).  The tile is read in source order
 // (tile bits sorted by source position, so consecutive threads read consecutive source
 // amplitudes), kept in LDS in destination order, and written in destination order.
-constexpr unsigned kPermLow = 5;
+// kPermLow: 512-byte runs, 5 bits for complex128 and 6 for complex64 (32-qubit complex64
+// with 5 bits, i.e. 256-byte runs: 2.35 TB/s for every permutation).
+template <typename T>
+constexpr unsigned perm_low() { return sizeof(T) == 8 ? 5 : 6; }
 struct PermTile {
   unsigned m;          // tile bits
   unsigned tb[16];     // destination bits of the tile, ascending
@@ -913,7 +937,7 @@ __global__ __launch_bounds__(kBlock) void k_permute_tiled(typename Vec<T>::t* __
                                                           const typename Vec<T>::t* __restrict__ src, PermTile pt)
 {
   using V = typename Vec<T>::t;
-  __shared__ V tile[1u << (2 * kPermLow)];
+  __shared__ V tile[1u << (2 * perm_low<T>())];
   const uint64_t t = block_id();
   uint64_t j0 = 0, s0 = 0;
   for (unsigned i = 0; i < pt.nout; ++i)

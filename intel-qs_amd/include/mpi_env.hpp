@@ -37,6 +37,7 @@ class Environment
   {
     if (num_states != 1)
       throw std::invalid_argument("one process holds one state here; a pool of states is iqs::StatePool (state_pool.hpp)");
+    std::printf("MPI not enabled.\n");  // as the reference's single-process build (src/mpi_env.cpp:43-44)
   }
   template <class T>
   static T IncoherentSumOverAllStatesOfPool(T v) { return v; }

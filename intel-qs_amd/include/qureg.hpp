@@ -275,6 +275,7 @@ class QubitRegister
   Type* host_() const;       // host mirror, downloaded if stale; marks it modified
   unsigned pos_(unsigned qubit) const;
   void enqueue_(unsigned npos, unsigned p0, unsigned p1, const Type* m);
+  void nest_(const char* who, std::size_t sind, std::size_t eind, const uint64_t c[9], const double md[8], double frac);
   void gate1_(unsigned position, const Type m[4], GateSpec1Q spec, BaseType angle);
   void cgate1_(unsigned cpos, unsigned tpos, const Type m[4], GateSpec2Q spec, BaseType angle);
   BaseType expval_(const std::vector<unsigned>& qubits, const std::vector<unsigned>& obs, BaseType coeff);

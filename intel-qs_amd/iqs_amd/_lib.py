@@ -51,6 +51,8 @@ def _sig(L):
         "iqsh_init_uniform": [P, D, D],
         "iqsh_apply_1q": [P, U, P],
         "iqsh_apply_1q_spec": [P, U, I, D],
+        "iqsh_apply_nest": [P, P, P],
+        "iqsh_spec_matrix": [I, I, I, D, P],
         "iqsh_apply_c1q": [P, U, U, P],
         "iqsh_apply_c1q_spec": [P, U, U, I, D],
         "iqsh_apply_mc1q": [P, U64, U, P],

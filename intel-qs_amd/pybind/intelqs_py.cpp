@@ -208,6 +208,24 @@ void bind_register(py::module_& m, const char* name)
              a.ApplyControlled1QubitGate(c, q, tm2<T>(mm, "The shape of the unitary-matrix is not 2x2."));
            },
            "Apply custom controlled-1-qubit gate.")
+      // Extension: the C++ sub-range helpers (public in the reference's qureg.hpp, not bound
+      // by its module); spec is the GateSpec1Q / GateSpec2Q number (8 = None).
+      .def("Apply1QubitGate_helper",
+           [](Q& a, unsigned q, py::array_t<T, py::array::c_style | py::array::forcecast> mm, std::size_t sind,
+              std::size_t eind, int spec, double angle) {
+             return a.Apply1QubitGate_helper(q, tm2<T>(mm, "Input shape is not 2x2."), sind, eind,
+                                             (iqs::GateSpec1Q)spec, (typename Q::BaseType)angle);
+           },
+           py::arg("qubit"), py::arg("m"), py::arg("sind"), py::arg("eind"), py::arg("spec") = 8,
+           py::arg("angle") = 0.0)
+      .def("ApplyControlled1QubitGate_helper",
+           [](Q& a, unsigned c, unsigned q, py::array_t<T, py::array::c_style | py::array::forcecast> mm,
+              std::size_t sind, std::size_t eind, int spec, double angle) {
+             return a.ApplyControlled1QubitGate_helper(c, q, tm2<T>(mm, "Input shape is not 2x2."), sind, eind,
+                                                       (iqs::GateSpec2Q)spec, (typename Q::BaseType)angle);
+           },
+           py::arg("control"), py::arg("qubit"), py::arg("m"), py::arg("sind"), py::arg("eind"),
+           py::arg("spec") = 8, py::arg("angle") = 0.0)
       .def("ApplyDiag",
            [](Q& a, unsigned q1, unsigned q2, py::array_t<T, py::array::c_style | py::array::forcecast> mm) {
              a.ApplyDiag(q1, q2, tm4<T>(mm));

@@ -402,15 +402,47 @@ void QubitRegister<Type>::enqueue_(unsigned np, unsigned p0, unsigned p1, const 
   queue_->push_back(g);
 }
 
+// The 2x2 (as 8 doubles) a sub-range helper applies: the SpecializeV2 constants when that
+// path is on and the kind has a body, else the caller's matrix.
+template <class Type>
+static void range_matrix(const Type mm[4], bool spec_on, int spec, bool controlled, double angle, double md[8])
+{
+  if (spec_on) ck(iqsh_spec_matrix(dtype_of<Type>(), spec, controlled ? 1 : 0, angle, md));
+  else to_doubles(mm, 4, md);
+}
+
+template <class Type>
+void QubitRegister<Type>::nest_(const char* who, std::size_t sind, std::size_t eind, const uint64_t c[9],
+                                const double md[8], double frac)
+{
+  if (sind > eind || eind > global_size_)
+    throw std::invalid_argument(std::string(who) + ": index range outside the register");
+  iqsh_state* h = dev_();
+  if (timer) timer->Start(std::string(who), 0);
+  ck(iqsh_apply_nest(h, c, md));
+  if (timer) {
+    ck(iqsh_sync());
+    timer->Stop(2.0 * frac * double(eind - sind) * sizeof(Type));
+  }
+}
+
 template <class Type>
 bool QubitRegister<Type>::Apply1QubitGate_helper(unsigned qubit, TM2x2<Type> const& m, std::size_t sind,
                                                  std::size_t eind, GateSpec1Q spec, BaseType angle)
 {
-  if (sind != 0 || eind != global_size_)
-    throw std::invalid_argument("Apply1QubitGate_helper: only the full index range is supported");
   Type mm[4];
   m2(m, mm);
-  gate1_(pos_(qubit), mm, spec, angle);
+  if (sind == 0 && eind == global_size_) {
+    gate1_(pos_(qubit), mm, spec, angle);
+    return true;
+  }
+  // Loop_DN over [sind, eind) (src/qureg_apply1qubitgate.cpp:196-202): groups of 2^(P+1)
+  // from sind, pairs (i, i + 2^P) in the first half of each.
+  const uint64_t P = pos_(qubit);
+  double md[8];
+  range_matrix(mm, specialize2 && spec != GateSpec1Q::None, (int)spec, false, (double)angle, md);
+  const uint64_t c[9] = {sind, eind, uint64_t(2) << P, 0, 1, 1, 0, uint64_t(1) << P, uint64_t(1) << P};
+  nest_("Apply1QubitGate_helper", sind, eind, c, md, 1.0);
   return true;
 }
 
@@ -434,12 +466,35 @@ bool QubitRegister<Type>::ApplyControlled1QubitGate_helper(unsigned c, unsigned 
                                                            std::size_t sind, std::size_t eind, GateSpec2Q spec,
                                                            BaseType angle)
 {
-  if (sind != 0 || eind != global_size_)
-    throw std::invalid_argument("ApplyControlled1QubitGate_helper: only the full index range is supported");
   if (c == t) throw std::invalid_argument("control and target qubit coincide");
   Type mm[4];
   m2(m, mm);
-  cgate1_(pos_(c), pos_(t), mm, spec, angle);
+  if (sind == 0 && eind == global_size_) {
+    cgate1_(pos_(c), pos_(t), mm, spec, angle);
+    return true;
+  }
+  // The reference's sub-range nests (src/qureg_applyctrl1qubitgate.cpp:262-346; every
+  // position is local on one GPU).
+  const uint64_t C = pos_(c), T = pos_(t), one = 1;
+  const bool spec_on = specialize2 && spec != GateSpec2Q::None;
+  double md[8];
+  if (C > T && C >= log2llc && global_size_ > eind - sind) {
+    // control stride beyond the cache block: the whole block has bit C = 0 or 1
+    if (!((sind >> C) & 1)) return false;
+    const GateSpec1Q s1 = ConvertSpec2to1(spec);
+    range_matrix(mm, specialize2 && s1 != GateSpec1Q::None, (int)s1, false, (double)angle, md);
+    const uint64_t nest[9] = {sind, eind, one << (T + 1), 0, 1, 1, 0, one << T, one << T};
+    nest_("ApplyControlled1QubitGate_helper", sind, eind, nest, md, 1.0);
+    return true;
+  }
+  range_matrix(mm, spec_on, (int)spec, true, (double)angle, md);
+  if (C > T) {
+    const uint64_t nest[9] = {sind, eind, one << (C + 1), one << C, one << (C + 1), one << (T + 1), 0, one << T, one << T};
+    nest_("ApplyControlled1QubitGate_helper", sind, eind, nest, md, 0.5);
+  } else {
+    const uint64_t nest[9] = {sind, eind, one << (T + 1), 0, one << T, one << (C + 1), one << C, one << (C + 1), one << T};
+    nest_("ApplyControlled1QubitGate_helper", sind, eind, nest, md, 0.5);
+  }
   return true;
 }
 
@@ -1070,6 +1125,15 @@ typename QubitRegister<Type>::BaseType QubitRegister<Type>::expval_(const std::v
   }
   double v, n2;
   ck(iqsh_expval_pauli(dev_(), x, z, ny, &v, &n2));
+  // The reference rotates each X or Y factor into the Z basis and back with two one-qubit
+  // gates (ApplyHadamard / Apply1QubitGate(G), qureg_expectval.cpp:146-156, 194-206); its
+  // gate counter sees them, so it sees them here too.
+  if (gate_counter)
+    for (std::size_t i = 0; i < qs.size(); ++i)
+      if (os[i] == 1 || os[i] == 2) {
+        gate_counter->OneQubitIncrement(qs[i]);
+        gate_counter->OneQubitIncrement(qs[i]);
+      }
   if (qs.size() == 1) {
     // 1 - 2 P(1) in the rotated basis = 1 - |psi|^2 + <P>  (qureg_expectval.cpp:18-74)
     const BaseType e = (BaseType)(1.0 - n2 + v);

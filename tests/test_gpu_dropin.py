@@ -297,3 +297,70 @@ def test_qubit_register_metric_counts_and_depth():
     assert (m.GetOneQubitGateCount(), m.GetTwoQubitGateCount(), m.GetTotalQubitGateCount()) == (5, 3, 8)
     assert m.GetParallelDepth() == 5
     assert np.array_equal(np.array(m, copy=True), np.array(p, copy=True))
+
+
+# ---- sub-range gate helpers (src/qureg_apply1qubitgate.cpp:173-230,
+#      src/qureg_applyctrl1qubitgate.cpp:228-346) ----
+
+def _loop_nest(s, c11, c12, c13, c21, c22, c23, c31, c32, shift, m):
+    """Loop_TN's nest (src/highperfkernels.cpp:397-486) as the reference runs it."""
+    s = s.copy()
+    for i in range(c11, c12, c13):
+        for j in range(i + c21, i + c22, c23):
+            for k in range(j + c31, j + c32):
+                a, b = s[k], s[k + shift]
+                s[k], s[k + shift] = m[0, 0] * a + m[0, 1] * b, m[1, 0] * a + m[1, 1] * b
+    return s
+
+
+def _ref_1q_helper(s, P, sind, eind, m):
+    return _loop_nest(s, sind, eind, 2 << P, 0, 1, 1, 0, 1 << P, 1 << P, m)
+
+
+def _ref_c1q_helper(s, C, T, sind, eind, m, log2llc):
+    if C > T and C >= log2llc and s.size > eind - sind:
+        return _ref_1q_helper(s, T, sind, eind, m) if (sind >> C) & 1 else s
+    if C > T:
+        return _loop_nest(s, sind, eind, 2 << C, 1 << C, 2 << C, 2 << T, 0, 1 << T, 1 << T, m)
+    return _loop_nest(s, sind, eind, 2 << T, 0, 1 << T, 2 << C, 1 << C, 2 << C, 1 << T, m)
+
+
+@pytest.mark.parametrize("cls", ["QubitRegister", "QubitRegisterSP"])
+def test_sub_range_helpers(cls):
+    n = 10
+    dt = np.complex128 if cls == "QubitRegister" else np.complex64
+    tol = 1e-13 if cls == "QubitRegister" else 2e-6
+    r = np.random.default_rng(3)
+    h = (r.normal(size=1 << n) + 1j * r.normal(size=1 << n)).astype(dt)
+    h /= np.linalg.norm(h)
+    m = np.array([[0.6, 0.8j], [0.8j, 0.6]], dtype=np.complex128)
+
+    def fresh():
+        reg = getattr(iqs, cls)(n, "base", 0, 0)
+        np.array(reg, copy=False)[:] = h
+        return reg
+
+    cases_1q = [(0, 0, 64), (3, 128, 512), (6, 256, 1024), (2, 40, 200), (9, 0, 1024), (4, 992, 1024)]
+    for P, sind, eind in cases_1q:
+        reg = fresh()
+        assert reg.Apply1QubitGate_helper(P, m.astype(dt), sind, eind)
+        want = _ref_1q_helper(h.astype(np.complex128), P, sind, eind, m)
+        assert np.abs(np.array(reg, copy=True) - want).max() < tol, (P, sind, eind)
+    # control above the cache block (log2llc = 5 after TurnOnFusion(5)), below it, and C < T
+    for C, T, sind, eind in [(8, 2, 256, 288), (8, 2, 0, 32), (7, 3, 128, 256), (3, 6, 0, 512), (1, 0, 100, 300),
+                             (9, 4, 0, 1024)]:
+        reg = fresh()
+        reg.TurnOnFusion(5)
+        reg.TurnOffFusion()  # keeps log2llc = 5, fusion off
+        reg.ApplyControlled1QubitGate_helper(C, T, m.astype(dt), sind, eind)
+        want = _ref_c1q_helper(h.astype(np.complex128), C, T, sind, eind, m, 5)
+        assert np.abs(np.array(reg, copy=True) - want).max() < tol, (C, T, sind, eind)
+    # SpecializeV2 constants on a sub-range: RX(0.7) through the spec path equals the matrix
+    reg = fresh()
+    reg.TurnOnSpecializeV2()
+    reg.Apply1QubitGate_helper(5, np.eye(2, dtype=dt), 64, 512, 1, 0.7)  # GateSpec1Q::RotationX
+    c, s_ = np.cos(0.35), np.sin(0.35)
+    rx = np.array([[c, -1j * s_], [-1j * s_, c]])
+    assert np.abs(np.array(reg, copy=True) - _ref_1q_helper(h.astype(np.complex128), 5, 64, 512, rx)).max() < tol
+    with pytest.raises(Exception):
+        fresh().Apply1QubitGate_helper(3, m.astype(dt), 0, 2048)  # beyond the register

@@ -162,6 +162,16 @@ int iqsh_norm2(iqsh_handle h, double *out);
  * (src/qureg_expectval.cpp:102-213). */
 int iqsh_expval_pauli(iqsh_handle h, uint64_t xmask, uint64_t zmask, unsigned ny,
                       double *value, double *norm2);
+/* iqsh_expval_pauli without any allocation: no counterpart in the reference; for callers
+ * that cannot allocate device memory (tight serving loops, code that forbids hipMalloc).
+ * Same kernels and summation order as iqsh_expval_pauli (identical results); its partial
+ * sums live in a fixed 64 KiB device array per GPU (at most 2048 partial sums per value,
+ * which covers every register size: the reduction grid is capped there), whose address is
+ * taken on the first call on each device.  Every call on a GPU uses that GPU's one array
+ * and leaves its partial sums there; calls that overlapped on one GPU would race on it,
+ * so calls take turns (a lock held until the result is back on the host). */
+int iqsh_expval_pauli_noalloc(iqsh_handle h, uint64_t xmask, uint64_t zmask, unsigned ny,
+                              double *value, double *norm2);
 /* <x|h> = sum conj(x_i) h_i (ComputeOverlap, src/qureg_utils.cpp:267-300). */
 int iqsh_overlap(iqsh_handle h, iqsh_handle x, double *re, double *im);
 /* max |h_i - s x_i| (MaxAbsDiff, src/qureg_utils.cpp:40-68); s = (sre, sim). */

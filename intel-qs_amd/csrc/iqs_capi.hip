@@ -1593,6 +1593,64 @@ int iqsh_expval_pauli(iqsh_handle h, uint64_t xmask, uint64_t zmask, unsigned ny
   return IQSH_OK;
 }
 
+// ---- the expectation value without allocation (iqsh_expval_pauli_noalloc) ----
+// Its partial sums and result live in `scratch`, a device array of fixed size (one copy
+// per GPU, held until the process ends): kMaxRedVals rows of kMaxRedBlocks partial sums,
+// then kMaxRedVals results.  Host code never uses the array's name as an address: on its
+// first call on a device, with that device current, the entry point takes the device
+// address with hipGetSymbolAddress and keeps it for that device.
+namespace {
+constexpr size_t kScratchDoubles = size_t(kMaxRedVals) * kMaxRedBlocks + kMaxRedVals;  // 64 KiB + 32 B
+__device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+double* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;  // calls of this process take turns on the one array per GPU
+
+int scratch_ptr(double** out)
+{
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(IQSH_ERR_NOTSUP, "device %d beyond %d", dev, kMaxDevices);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return IQSH_OK;
+}
+
+template <typename T>
+int do_pauli_noalloc(iqsh_handle h, uint64_t x, uint64_t z, unsigned ny, double* out2)
+{
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  double* scr;
+  if (int rc = scratch_ptr(&scr)) return rc;
+  const unsigned g = red_grid(x ? h->size() / 2 : h->size());
+  if (g > kMaxRedBlocks) return fail(IQSH_ERR_ARG, "%u partial sums exceed the scratch array's %d", g, kMaxRedBlocks);
+  double* result = scr + size_t(kMaxRedVals) * kMaxRedBlocks;
+  k_red_pauli<T><<<g, kBlock, 0, g_stream>>>((const typename Vec<T>::t*)h->d, h->size(), x, z, ny, scr);
+  CHECK_LAUNCH();
+  k_red_final<<<1, kBlock, 0, g_stream>>>(scr, g, 2, 0, result);
+  CHECK_LAUNCH();
+  HIP_TRY(hipMemcpyAsync(out2, result, 2 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+  HIP_TRY(hipStreamSynchronize(g_stream));
+  return IQSH_OK;
+}
+}  // namespace
+
+int iqsh_expval_pauli_noalloc(iqsh_handle h, uint64_t xmask, uint64_t zmask, unsigned ny, double* value, double* nrm2)
+{
+  if (int rc = check_h(h)) return rc;
+  if ((xmask | zmask) >> h->n) return fail(IQSH_ERR_ARG, "Pauli mask exceeds register");
+  if ((unsigned)__builtin_popcountll(xmask & zmask) != ny) return fail(IQSH_ERR_ARG, "ny must equal popcount(xmask & zmask)");
+  double r[2];
+  if (int rc = DISPATCH(h, do_pauli_noalloc)(h, xmask, zmask, ny, r)) return rc;
+  if (value) *value = r[0];
+  if (nrm2) *nrm2 = r[1];
+  return IQSH_OK;
+}
+
 static int two_check(iqsh_handle h, iqsh_handle x)
 {
   if (int rc = check_h(h)) return rc;

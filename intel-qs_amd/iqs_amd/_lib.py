@@ -69,6 +69,7 @@ def _sig(L):
         "iqsh_norm2": [P, DP],
         "iqsh_maxnorm_half": [P, U, I, DP],
         "iqsh_expval_pauli": [P, U64, U64, U, DP, DP],
+        "iqsh_expval_pauli_noalloc": [P, U64, U64, U, DP, DP],
         "iqsh_overlap": [P, P, DP, DP],
         "iqsh_maxabsdiff": [P, P, D, D, DP],
         "iqsh_l2diff2": [P, P, DP],

@@ -100,6 +100,17 @@ def traffic_from_profile(n: int):
     return None
 
 
+def profile_value(name: str, key: str):
+    """A `key: value` line of a committed profiles/ summary, or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(key + ":"):
+            return float(line.split(":")[1])
+    return None
+
+
 def calibrate(n: int) -> int:
     """Run one device-to-device copy of a known size (for PMC counter calibration)."""
     from iqs_amd.engine import StateVector, sync
@@ -657,6 +668,14 @@ def batch_measure(reps: int, torch, tune=None):
     out["c64"] = {"ms_forward_and_gradient": ms_grad, "ms_forward_only": ms_fwd,
                   "instances_per_s": len(nq) / (ms_grad * 1e-3),
                   "algorithmic_bytes": traffic, "algorithmic_GBs": traffic / (ms_grad * 1e-3) / 1e9}
+    # Bound of the persistent kernel, from the committed counter summary: VALU issue (busy
+    # fraction of the tuned complex64 gradient run) and its measured HBM traffic.
+    busy = profile_value("k_hea_batch512_pmc.md", "valu_busy_c64_grad")
+    hbm = profile_value("k_hea_batch512_pmc.md", "traffic_bytes_c64_grad")
+    out["c64"]["roofline"] = {"bound": "valu", "valu_busy_frac": busy,
+                              "hbm_traffic_bytes": hbm,
+                              "hbm_frac": hbm / (ms_grad * 1e-3) / (HBM_PEAK_GBS * 1e9) if hbm else None,
+                              "source": "profiles/k_hea_batch512_pmc.md (rocprofv3 counters of the same tuned run)"}
     log("HEA batch: complex128 reference run")
     b128 = HeaBatch(nq, dp, "c128")
     c128, g128 = b128.evaluate(ang)

@@ -513,7 +513,11 @@ __global__ __launch_bounds__(kBlock) void k_red_pauli(const typename Vec<T>::t* 
       v[1] += p;
     }
   } else {
-    const unsigned lsb = __ffsll((long long)xmask) - 1;
+    // Pairs (j, j ^ xmask) enumerated with a zero inserted at the highest X/Y position: both
+    // j and j ^ xmask then run through whole cache lines across a wavefront.  (Inserting it
+    // at the lowest position made each access half a line whose other half is read far
+    // away: 2.9 TB/s for X on positions 2 and 26 at 30 qubits, against 5.6 for one X.)
+    const unsigned msb = 63 - __clzll((long long)xmask);
     // <P> = sum_k conj(psi_{k^x}) psi_k i^ny (-1)^popc(k & z).  For the pair (j, j^x) with
     // a = psi_j, b = psi_{j^x}, t = conj(a) b, s = (-1)^popc(j & z):
     //   term = i^ny s [conj(t) + (-1)^ny t]
@@ -522,7 +526,7 @@ __global__ __launch_bounds__(kBlock) void k_red_pauli(const typename Vec<T>::t* 
     const double sgn_even = ((ny / 2) & 1) ? -1.0 : 1.0;
     const double sgn_odd = (((ny - 1) / 2) & 1) ? -1.0 : 1.0;
     for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < N / 2; k += stride) {
-      const uint64_t j = ins0(k, lsb), jp = j ^ xmask;
+      const uint64_t j = ins0(k, msb), jp = j ^ xmask;
       const auto a = st[j], b = st[jp];
       const double tr = (double)a.x * b.x + (double)a.y * b.y;  // Re conj(a) b
       const double ti = (double)a.x * b.y - (double)a.y * b.x;  // Im conj(a) b

@@ -110,62 +110,19 @@ __global__ __launch_bounds__(kLB) void k_sum_part(const double* __restrict__ par
   wg_sum_store(v, out, 0);
 }
 
-// x from lane ^ 2^q, in the VALU (no LDS round trip).  Within a 16-lane row (q < 4) DPP
-// moves: quad permutations for q = 0, 1; row_half_mirror (lane i -> 7 - i in each half-row)
-// followed by the quad permutation i -> i ^ 3 gives i ^ 4; row_mirror (i -> 15 - i) followed
-// by row_half_mirror gives i ^ 8.  Across rows, gfx950's lane swaps of a value with itself:
-// v_permlane16_swap exchanges rows 0 <-> 1 and 2 <-> 3 between its two results, so lanes
-// with bit 4 set take the first result and the others the second (i ^ 16);
-// v_permlane32_swap likewise exchanges the wave's halves (i ^ 32).
-template <int CTRL>
-__device__ __forceinline__ uint32_t dpp32(uint32_t v)
-{
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, false);
-}
-
-__device__ __forceinline__ uint32_t xor_lane32(uint32_t v, int q)
-{
-  switch (q) {
-    case 0: return dpp32<0xB1>(v);                  // quad_perm [1, 0, 3, 2]
-    case 1: return dpp32<0x4E>(v);                  // quad_perm [2, 3, 0, 1]
-    case 2: return dpp32<0x1B>(dpp32<0x141>(v));   // row_half_mirror, quad_perm [3, 2, 1, 0]
-    case 3: return dpp32<0x141>(dpp32<0x140>(v));  // row_mirror, row_half_mirror
-    case 4: {
-      const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
-      return (threadIdx.x & 16) ? r[0] : r[1];
-    }
-    default: {
-      const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
-      return (threadIdx.x & 32) ? r[0] : r[1];
-    }
-  }
-}
-
-__device__ __forceinline__ float xor_lane(float v, int q)
-{
-  return __builtin_bit_cast(float, xor_lane32(__builtin_bit_cast(uint32_t, v), q));
-}
-
-__device__ __forceinline__ double xor_lane(double v, int q)
-{
-  const uint64_t b = __builtin_bit_cast(uint64_t, v);
-  const uint64_t lo = xor_lane32((uint32_t)b, q), hi = xor_lane32((uint32_t)(b >> 32), q);
-  return __builtin_bit_cast(double, (hi << 32) | lo);
-}
-
 template <typename V>
-__device__ __forceinline__ V shfl_v(const V& a, int q)
+__device__ __forceinline__ V shfl_v(const V& a, int mask)
 {
   V r;
-  r.x = xor_lane(a.x, q);
-  r.y = xor_lane(a.y, q);
+  r.x = __shfl_xor(a.x, mask);
+  r.y = __shfl_xor(a.y, mask);
   return r;
 }
 
-// Lane-shuffle pass over positions 0..5.  N a multiple of kLB * kLowP.
+// Lane-shuffle pass over positions 0..L-1 (L <= 6).  N a multiple of kLB * kLowP.
 template <typename T, bool ADJ>
 __global__ __launch_bounds__(kLB) void k_layer_low(typename Vec<T>::t* __restrict__ psi, typename Vec<T>::t* __restrict__ lam,
-                                                   uint64_t N, Mat2<T> u, Mat2<double> g, double* __restrict__ part)
+                                                   uint64_t N, int L, Mat2<T> u, Mat2<double> g, double* __restrict__ part)
 {
   using V = typename Vec<T>::t;
   const int lane = threadIdx.x & 63;
@@ -184,19 +141,16 @@ __global__ __launch_bounds__(kLB) void k_layer_low(typename Vec<T>::t* __restric
       x[j] = ldg(&psi[r0 + (uint64_t)j * kLB]);
       if (ADJ) y[j] = ldg(&lam[r0 + (uint64_t)j * kLB]);
     }
-    // Forward: unrolled (each position's lane moves inline).  Adjoint: a loop, the position
-    // switch in xor_lane32 picking the move (unrolled, psi and lambda of all positions in
-    // flight took 170-256 VGPRs).
-#pragma unroll(ADJ ? 1 : kLowQ)
-    for (int q = 0; q < kLowQ; ++q) {
+    for (int q = 0; q < L; ++q) {
+      const int msk = 1 << q;
       const bool hi = (lane >> q) & 1;
 #pragma unroll
       for (int j = 0; j < kLowP; ++j) {
-        const V xp = shfl_v(x[j], q);
+        const V xp = shfl_v(x[j], msk);
         const V x0 = hi ? xp : x[j], x1 = hi ? x[j] : xp;
         if (ADJ) {
           acc += hi ? im_term(y[j], g.m10, x0, g.m11, x1) : im_term(y[j], g.m00, x0, g.m01, x1);
-          const V yp = shfl_v(y[j], q);
+          const V yp = shfl_v(y[j], msk);
           const V y0 = hi ? yp : y[j], y1 = hi ? y[j] : yp;
           y[j] = hi ? mat_row(u.m10, y0, u.m11, y1) : mat_row(u.m00, y0, u.m01, y1);
         }
@@ -226,7 +180,7 @@ __global__ __launch_bounds__(kLB) void k_layer_low(typename Vec<T>::t* __restric
 // plain lane pass.  W: weighted cuts, summed from byte tables of weight sums in LDS.
 template <typename T, bool ADJ, bool W>
 __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict__ psi, typename Vec<T>::t* __restrict__ lam,
-                                                  uint64_t N, Mat2<T> u, Mat2<double> g, CutSpec cs,
+                                                  uint64_t N, int L, Mat2<T> u, Mat2<double> g, CutSpec cs,
                                                   double* __restrict__ part)
 {
   using V = typename Vec<T>::t;
@@ -286,19 +240,16 @@ __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict
       }
       if (!ADJ) x[j] = cmul(f[j], x[j]);
     }
-    // Forward: unrolled (each position's lane moves inline).  Adjoint: a loop, the position
-    // switch in xor_lane32 picking the move (unrolled, psi and lambda of all positions in
-    // flight took 170-256 VGPRs).
-#pragma unroll(ADJ ? 1 : kLowQ)
-    for (int q = 0; q < kLowQ; ++q) {
+    for (int q = 0; q < L; ++q) {
+      const int msk = 1 << q;
       const bool hi = (lane >> q) & 1;
 #pragma unroll
       for (int j = 0; j < P; ++j) {
-        const V xp = shfl_v(x[j], q);
+        const V xp = shfl_v(x[j], msk);
         const V x0 = hi ? xp : x[j], x1 = hi ? x[j] : xp;
         if (ADJ) {
           acc += hi ? im_term(y[j], g.m10, x0, g.m11, x1) : im_term(y[j], g.m00, x0, g.m01, x1);
-          const V yp = shfl_v(y[j], q);
+          const V yp = shfl_v(y[j], msk);
           const V y0 = hi ? yp : y[j], y1 = hi ? y[j] : yp;
           y[j] = hi ? mat_row(u.m10, y0, u.m11, y1) : mat_row(u.m00, y0, u.m01, y1);
         }
@@ -476,6 +427,7 @@ int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm,
   if (ADJ) g = Mat2<double>{{gm[0], gm[1]}, {gm[2], gm[3]}, {gm[4], gm[5]}, {gm[6], gm[7]}};
   double acc = 0, accg = 0;
   const unsigned n = h->n;
+  const int L = (int)std::min<unsigned>(n, kLowQ);
   // Forward: the lane pass first (it carries the phase layer, applied before the mixer);
   // adjoint: last (the phase is undone after the whole mixer).
   auto low_pass = [&]() -> int {
@@ -488,11 +440,11 @@ int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm,
       if (int rc = part_buf(2 * (uint64_t)grid + 2)) return rc;
     V* lp = ADJ ? (V*)lam->d : nullptr;
     if (cs && cs->weighted)
-      k_qaoa_low<T, ADJ, true><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, h->size(), u, g, *cs, g_lw.part);
+      k_qaoa_low<T, ADJ, true><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, h->size(), L, u, g, *cs, g_lw.part);
     else if (cs)
-      k_qaoa_low<T, ADJ, false><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, h->size(), u, g, *cs, g_lw.part);
+      k_qaoa_low<T, ADJ, false><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, h->size(), L, u, g, *cs, g_lw.part);
     else
-      k_layer_low<T, ADJ><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, ADJ ? (V*)lam->d : nullptr, h->size(), u, g,
+      k_layer_low<T, ADJ><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, ADJ ? (V*)lam->d : nullptr, h->size(), L, u, g,
                                                              g_lw.part);
     IQSI_TRY(hipGetLastError());
     if (!ADJ) return IQSH_OK;
@@ -525,7 +477,6 @@ int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm,
 
 // The lane pass needs whole rows of kLB * kLowP amplitudes with lanes spanning >= 64.
 constexpr unsigned kMinLayerQubits = 10;
-static_assert(kMinLayerQubits >= kLowQ, "the lane pass always takes positions 0..kLowQ-1");
 
 }  // namespace
 

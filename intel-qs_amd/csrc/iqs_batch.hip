@@ -62,7 +62,7 @@ struct HeaInst {
   uint32_t ntl;     // tiled: log2(tiles) = n - tile qubits
   uint64_t psi, lam;  // amplitude offsets of the two vectors in the arena
   uint64_t par;       // first (theta, phi) pair of the instance
-  uint64_t part;      // tiled: gradient partials [depth][n][3][tiles]
+  uint64_t part;      // tiled: gradient partials [depth][n][2 (theta, phi)][tiles]
   uint64_t cpart;     // tiled: cost partials [tiles]
   uint32_t arrive;    // tiled: arrival counter of step 0 (one counter per step)
   uint32_t pad;
@@ -159,31 +159,6 @@ struct Geo {
   }
 };
 
-template <typename C>
-__device__ __forceinline__ void u_apply(const C* u, C& a, C& b)
-{
-  C r0, r1;
-  r0.x = (u[0].x * a.x - u[0].y * a.y) + (u[1].x * b.x - u[1].y * b.y);
-  r0.y = (u[0].x * a.y + u[0].y * a.x) + (u[1].x * b.y + u[1].y * b.x);
-  r1.x = (u[2].x * a.x - u[2].y * a.y) + (u[3].x * b.x - u[3].y * b.y);
-  r1.y = (u[2].x * a.y + u[2].y * a.x) + (u[3].x * b.y + u[3].y * b.x);
-  a = r0;
-  b = r1;
-}
-
-// U^dagger: rows (conj u00, conj u10), (conj u01, conj u11).
-template <typename C>
-__device__ __forceinline__ void ud_apply(const C* u, C& a, C& b)
-{
-  C r0, r1;
-  r0.x = (u[0].x * a.x + u[0].y * a.y) + (u[2].x * b.x + u[2].y * b.y);
-  r0.y = (u[0].x * a.y - u[0].y * a.x) + (u[2].x * b.y - u[2].y * b.x);
-  r1.x = (u[1].x * a.x + u[1].y * a.y) + (u[3].x * b.x + u[3].y * b.y);
-  r1.y = (u[1].x * a.y - u[1].y * a.x) + (u[3].x * b.y - u[3].y * b.x);
-  a = r0;
-  b = r1;
-}
-
 // Coherence of tile passes.  All tiles of one register are produced and consumed by
 // workgroups of the same XCD (one queue per XCD, chosen by the XCC_ID hardware register):
 // a producer waits for its stores to complete before bumping the arrival counter, and a
@@ -219,14 +194,43 @@ __device__ __forceinline__ double wave_sum63(double v)
   return v;
 }
 
+template <typename C>
+__device__ __forceinline__ C cx_mul(const C& a, const C& b)
+{
+  C r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
+}
+
+template <typename C>
+__device__ __forceinline__ C cx_mulc(const C& a, const C& b)  // a * conj(b)
+{
+  C r;
+  r.x = a.x * b.x + a.y * b.y;
+  r.y = a.y * b.x - a.x * b.y;
+  return r;
+}
+
+// Layer split.  U_l = CZ_l . prod_q RZ_q(phi_lq) . prod_q RY_q(theta_lq): the RYs are real
+// 2x2 rotations (half the arithmetic of the complex U = RZ RY per pair), and every RZ and CZ
+// factor of a register phase is one diagonal applied per amplitude.  An RZ commutes with
+// everything between its qubit's RY of the same layer and that qubit's RY of the next, so
+// the RZs of the qubits a pass rotates last (its RY_j) are deferred to the next pass's first
+// diagonal.  Gradients: dC/dtheta = Im<lam|Y_q|psi> just after the RY (one sum per pair),
+// dC/dphi = Im<lam|Z_q|psi> at the diagonal that holds the RZ (one product per amplitude,
+// signed by the qubit's bit), both from partial sums over a thread's registers.
+// mats: 4 entries per (layer, qubit): (cos theta/2, sin theta/2), (cos phi/2, sin phi/2), 0, 0.
 template <typename T, int M, int R>
 struct Tile {
   static constexpr int LR = ilog2(R);
   static constexpr int NT = (1 << M) / R;
   static constexpr int NW = NT / 64;
-  static constexpr int SLOTS = 6 * M;  // two layers x up to M gated bits x 3 sums
+  static constexpr int KMAX = M;       // register phases of one sweep (at most)
+  static constexpr int SLOTS = 4 * M;  // dtheta of two layers' gated bits, dphi of <= 2M qubits
   static_assert(R >= 4 && (R & (R - 1)) == 0, "R must be a power of two >= 4");
   static_assert(NT >= 64 && NT <= 1024, "workgroup of 1..16 wavefronts");
+  static_assert(KMAX * R <= NT, "one table entry per thread");
   using G = Geo<M, LR>;
   using C = V<T>;
 
@@ -241,77 +245,171 @@ struct Tile {
     __syncthreads();
   }
 
-  // Gates U_q (forward) on the register bits in rmask; `row` = matrices of qubit 0 of a layer.
-  static __device__ void gates_fwd(C* x, uint32_t rmask, const C* row, int a, const G& g)
+  // RY_q (real) on the register bits in rmask; `row` = matrices of qubit 0 of the layer.
+  static __device__ void ry_fwd(C* x, uint32_t rmask, const C* row, int a, const G& g)
   {
 #pragma unroll
     for (int j = 0; j < LR; ++j) {
       if ((rmask >> j) & 1u) {
-        const C* u = row + 4 * g.gq(a + j);
-        const C U[4] = {u[0], u[1], u[2], u[3]};
+        const C cs = row[4 * g.gq(a + j)];
+        const T c = cs.x, sn = cs.y;
 #pragma unroll
-        for (int k = 0; k < R; ++k)
-          if (!((k >> j) & 1)) u_apply(U, x[k], x[k | (1 << j)]);
+        for (int k = 0; k < R; ++k) {
+          if ((k >> j) & 1) continue;
+          const int k1 = k | (1 << j);
+          const C p0 = x[k], p1 = x[k1];
+          x[k].x = c * p0.x - sn * p1.x;
+          x[k].y = c * p0.y - sn * p1.y;
+          x[k1].x = sn * p0.x + c * p1.x;
+          x[k1].y = sn * p0.y + c * p1.y;
+        }
       }
     }
   }
 
-  static __device__ void cz_apply(C* x, uint32_t czm, int a, const G& g)
-  {
-    if (!czm) return;
-    const uint32_t tid = threadIdx.x;
-#pragma unroll
-    for (int k = 0; k < R; ++k) {
-      const uint32_t xg = g.gidx(lidx<LR>(tid, k, a));
-      if (__popc(xg & (xg >> 1) & czm) & 1) x[k] = -x[k];
-    }
-  }
-
-  // Backward through U_q on the register bits in rmask, for psi (x) and lam (y): the three
-  // sums of each qubit are wave-reduced into redw[slot0 + 3 (lbit - la) + {0,1,2}].
-  static __device__ void gates_bwd(C* x, C* y, uint32_t rmask, const C* row, int a, const G& g, double* redw,
-                                   int slot0)
+  // Backward through RY_q for psi (x) and lam (y): Im<lam|Y_q|psi> (summed over this thread's
+  // pairs, then the wavefront) into redw[slot0 + lbit - la], then RY_q^T on both.
+  static __device__ void ry_bwd(C* x, C* y, uint32_t rmask, const C* row, int a, const G& g, double* redw, int slot0)
   {
     const bool lane63 = (threadIdx.x & 63) == 63;
 #pragma unroll
     for (int j = 0; j < LR; ++j) {
       if ((rmask >> j) & 1u) {
-        const C* u = row + 4 * g.gq(a + j);
-        const C U[4] = {u[0], u[1], u[2], u[3]};
-        double sx = 0, sy = 0, sz = 0;
+        const C cs = row[4 * g.gq(a + j)];
+        const T c = cs.x, sn = cs.y;
+        T acc = 0;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
           if ((k >> j) & 1) continue;
           const int k1 = k | (1 << j);
           const C p0 = x[k], p1 = x[k1], l0 = y[k], l1 = y[k1];
-          // conj(l0) p1, conj(l1) p0, Im conj(l0) p0, Im conj(l1) p1: products in the
-          // amplitude precision, one per-pair combination per sum accumulated in FP64.
-          const T c01r = l0.x * p1.x + l0.y * p1.y, c01i = l0.x * p1.y - l0.y * p1.x;
-          const T c10r = l1.x * p0.x + l1.y * p0.y, c10i = l1.x * p0.y - l1.y * p0.x;
-          const T c00i = l0.x * p0.y - l0.y * p0.x, c11i = l1.x * p1.y - l1.y * p1.x;
-          sx += (double)(c01i + c10i);
-          sy += (double)(c10r - c01r);
-          sz += (double)(c00i - c11i);
-          ud_apply(U, x[k], x[k1]);
-          ud_apply(U, y[k], y[k1]);
+          // Im(conj(l0) (-i p1) + conj(l1) (i p0)) = Re(conj(l1) p0) - Re(conj(l0) p1)
+          acc += (l1.x * p0.x + l1.y * p0.y) - (l0.x * p1.x + l0.y * p1.y);
+          x[k].x = c * p0.x + sn * p1.x;
+          x[k].y = c * p0.y + sn * p1.y;
+          x[k1].x = c * p1.x - sn * p0.x;
+          x[k1].y = c * p1.y - sn * p0.y;
+          y[k].x = c * l0.x + sn * l1.x;
+          y[k].y = c * l0.y + sn * l1.y;
+          y[k1].x = c * l1.x - sn * l0.x;
+          y[k1].y = c * l1.y - sn * l0.y;
         }
-        sx = wave_sum63(sx);
-        sy = wave_sum63(sy);
-        sz = wave_sum63(sz);
-        if (lane63) {
-          double* r = redw + slot0 + 3 * (a + j - g.la);
-          r[0] = sx;
-          r[1] = sy;
-          r[2] = sz;
-        }
+        const double sum = wave_sum63((double)acc);
+        if (lane63) redw[slot0 + a + j - g.la] = sum;
       }
     }
   }
 
-  // R_pre (layer row_pre), CZ (if cz), R_post (layer row_post) on the gated bits, phases
-  // 0..K-1.  A negative row skips that layer.
+  // Register bits of phase k whose RZ is in that phase's diagonal: the pre-rotated bits
+  // [nb, ne), and in phase 0 the deferred qubits that are register bits there.
+  static __device__ uint32_t zreg(const G& g, int k, bool defer_b)
+  {
+    const int a = g.a(k);
+    uint32_t m = bit_range(g.nb(k) - a, g.ne(k) - a);
+    // deferred qubits 0..M-1 of a B pass: local bits [0, c); in phase 0 those >= a are
+    // register bits (a = M - LR < c when n < M + LR)
+    if (k == 0 && defer_b) m |= bit_range(0, g.c - a);
+    return m & ((1u << LR) - 1u);
+  }
+
+  // Diagonal tables of one sweep: zt[k][r] = product of the RZ factors of phase k's
+  // register bits for register index r (bit 0 -> e^{-i phi/2}, bit 1 -> e^{+i phi/2}).
+  // Barriers on both sides: no thread still reads the previous sweep's table.
+  static __device__ void build_tables(C* zt, const G& g, const C* row, bool defer_b)
+  {
+    __syncthreads();
+    const uint32_t i = threadIdx.x;
+    if (i < (uint32_t)(g.K * R)) {
+      const int k = i / R, r = i % R, a = g.a(k);
+      const uint32_t m = zreg(g, k, defer_b);
+      C f{1, 0};
+      for (int j = 0; j < LR; ++j)
+        if ((m >> j) & 1u) {
+          const C h = row[4 * g.gq(a + j) + 1];
+          f = cx_mul(f, C{h.x, ((r >> j) & 1) ? h.y : -h.y});
+        }
+      zt[i] = f;
+    }
+    __syncthreads();
+  }
+
+  // This thread's factor of the deferred RZs on bits that are not register bits in phase 0
+  // (lane bits and tile bits).  defer: global qubits [q0, q1) of the previous pass.
+  static __device__ C deferred_factor(const G& g, const C* row, int q0, int q1)
+  {
+    const int a = g.a(0);
+    const uint32_t x0 = g.gidx(lidx<LR>(threadIdx.x, 0, a));  // register bits 0
+    C f{1, 0};
+    for (int q = q0; q < q1; ++q) {
+      // register bit in phase 0: the table has it (only local bits [a, a + LR) of a B tile)
+      if (g.kind == 2 && q < g.c && q >= a && q < a + LR) continue;
+      const C h = row[4 * q + 1];
+      f = cx_mul(f, C{h.x, ((x0 >> q) & 1u) ? h.y : -h.y});
+    }
+    return f;
+  }
+
+  // x *= diag (conj: x *= conj(diag), and y likewise when given); diag = CZ signs (czm) x
+  // zt[k][r] (x f0 in phase 0).
+  template <bool BOTH>
+  static __device__ void diag_apply(C* x, C* y, const C* ztk, const C& f0, bool use_f0, uint32_t czm, int a,
+                                    const G& g, bool conj)
+  {
+    const uint32_t tid = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      C f = ztk[k];
+      if (use_f0) f = cx_mul(f, f0);
+      if (czm) {
+        const uint32_t xg = g.gidx(lidx<LR>(tid, k, a));
+        if (__popc(xg & (xg >> 1) & czm) & 1) f = C{-f.x, -f.y};
+      }
+      x[k] = conj ? cx_mulc(x[k], f) : cx_mul(x[k], f);
+      if (BOTH) y[k] = conj ? cx_mulc(y[k], f) : cx_mul(y[k], f);
+    }
+  }
+
+  // Im<lam|Z_q|psi> for the RZ qubits of phase k into redw[slot0 + q] (register bits of
+  // zreg, and in phase 0 the deferred qubits [q0, q1) on lane or tile bits).
+  static __device__ void dphi(const C* x, const C* y, int k, const G& g, bool defer_b, int q0, int q1, double* redw,
+                              int slot0)
+  {
+    const bool lane63 = (threadIdx.x & 63) == 63;
+    const int a = g.a(k);
+    T tot = 0, hi[LR];  // hi[j]: registers with bit j set (Z = -1 there)
+#pragma unroll
+    for (int j = 0; j < LR; ++j) hi[j] = 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const T t = y[r].x * x[r].y - y[r].y * x[r].x;  // Im(conj(lam) psi)
+      tot += t;
+#pragma unroll
+      for (int j = 0; j < LR; ++j)
+        if ((r >> j) & 1) hi[j] += t;
+    }
+    const uint32_t m = zreg(g, k, defer_b);
+#pragma unroll
+    for (int j = 0; j < LR; ++j) {
+      if ((m >> j) & 1u) {
+        const double sum = wave_sum63((double)(tot - 2 * hi[j]));
+        if (lane63) redw[slot0 + g.gq(a + j)] = sum;
+      }
+    }
+    if (k == 0 && q1 > q0) {
+      const uint32_t x0 = g.gidx(lidx<LR>(threadIdx.x, 0, a));
+      for (int q = q0; q < q1; ++q) {
+        if (g.kind == 2 && q < g.c && q >= a && q < a + LR) continue;  // register bit: done above
+        const double sum = wave_sum63((double)(((x0 >> q) & 1u) ? -tot : tot));
+        if (lane63) redw[slot0 + q] = sum;
+      }
+    }
+  }
+
+  // Forward sweep: per phase, RY of layer row_pre on the pre bits, the diagonal (RZ of
+  // row_pre on those bits and the deferred qubits [q0, q1), CZ pairs), RY of row_post.
+  // A negative row skips that layer.  zt: tables built for row_pre.
   static __device__ void sweep_fwd(C* x, C* lds, int& cur, const G& g, const C* mats, int64_t row_pre,
-                                   int64_t row_post, bool cz)
+                                   int64_t row_post, const C* zt, const C& f0, bool has_f0)
   {
     for (int k = 0; k < g.K; ++k) {
       const int nb = g.nb(k), ne = g.ne(k), a = g.a(k);
@@ -319,21 +417,23 @@ struct Tile {
         relayout(x, lds, cur, a);
         cur = a;
       }
-      if (row_pre >= 0) gates_fwd(x, bit_range(nb - a, ne - a), mats + 4 * row_pre, a, g);
-      if (cz) cz_apply(x, g.czmask(k), a, g);
+      if (row_pre >= 0) {
+        ry_fwd(x, bit_range(nb - a, ne - a), mats + 4 * row_pre, a, g);
+        diag_apply<false>(x, x, zt + k * R, f0, k == 0 && has_f0, g.czmask(k), a, g, false);
+      }
       if (row_post >= 0) {
         const int lo = max(nb - 1, g.la), hi = (k == g.K - 1) ? g.lb : ne - 1;
-        gates_fwd(x, bit_range(lo - a, hi - a), mats + 4 * row_post, a, g);
+        ry_fwd(x, bit_range(lo - a, hi - a), mats + 4 * row_post, a, g);
       }
     }
   }
 
-  // Exact reverse of sweep_fwd on psi and lam, with the gradient sums: post-layer slots at
-  // 0, pre-layer slots at 3 (lb - la).
+  // Exact reverse of sweep_fwd on psi and lam with the gradient sums: dtheta of row_post at
+  // slots [0, M), of row_pre at [M, 2M) (by local bit - la), dphi of row_pre at 2M + qubit.
   static __device__ void sweep_bwd(C* x, C* y, C* lds, int& cur, const G& g, const C* mats, int64_t row_pre,
-                                   int64_t row_post, bool cz, double* redw)
+                                   int64_t row_post, const C* zt, const C& f0, bool has_f0, bool defer_b, int q0,
+                                   int q1, double* redw)
   {
-    const int g3 = 3 * (g.lb - g.la);
     for (int k = g.K - 1; k >= 0; --k) {
       const int nb = g.nb(k), ne = g.ne(k), a = g.a(k);
       if (cur != a) {
@@ -343,14 +443,13 @@ struct Tile {
       }
       if (row_post >= 0) {
         const int lo = max(nb - 1, g.la), hi = (k == g.K - 1) ? g.lb : ne - 1;
-        gates_bwd(x, y, bit_range(lo - a, hi - a), mats + 4 * row_post, a, g, redw, 0);
+        ry_bwd(x, y, bit_range(lo - a, hi - a), mats + 4 * row_post, a, g, redw, 0);
       }
-      if (cz) {
-        const uint32_t m = g.czmask(k);
-        cz_apply(x, m, a, g);
-        cz_apply(y, m, a, g);
+      if (row_pre >= 0) {
+        dphi(x, y, k, g, defer_b, q0, q1, redw, 2 * M);
+        diag_apply<true>(x, y, zt + k * R, f0, k == 0 && has_f0, g.czmask(k), a, g, true);
+        ry_bwd(x, y, bit_range(nb - a, ne - a), mats + 4 * row_pre, a, g, redw, M);
       }
-      if (row_pre >= 0) gates_bwd(x, y, bit_range(nb - a, ne - a), mats + 4 * row_pre, a, g, redw, g3);
     }
   }
 
@@ -377,7 +476,7 @@ struct Tile {
     return tot;
   }
 
-  static __device__ void run_whole(const HeaArgs<T>& A, const HeaInst& I, uint32_t b, C* lds, double* red,
+  static __device__ void run_whole(const HeaArgs<T>& A, const HeaInst& I, uint32_t b, C* lds, C* zt, double* red,
                                    double* redc)
   {
     const uint32_t tid = threadIdx.x;
@@ -386,30 +485,36 @@ struct Tile {
     G g;
     g.init(n, 0, 0, n, 0, 0);
     C x[R], y[R];
+    const C one{1, 0};
     int cur = g.a(0);
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       x[k] = C{0, 0};
       if (lidx<LR>(tid, k, cur) == 0) x[k].x = 1;
     }
-    for (int l = 0; l < D; ++l) sweep_fwd(x, lds, cur, g, A.mats, (int64_t)(I.par + (uint64_t)l * n), -1, true);
+    for (int l = 0; l < D; ++l) {
+      const uint64_t row = I.par + (uint64_t)l * n;
+      build_tables(zt, g, A.mats + 4 * row, false);
+      sweep_fwd(x, lds, cur, g, A.mats, (int64_t)row, -1, zt, one, false);
+    }
     const double cost = cost_and_lam(x, y, grad, cur, g, redc);
     if (tid == 0) A.cost[b] = cost;
     if (grad) {
       double* redw = red + (tid >> 6) * SLOTS;
       for (int l = D - 1; l >= 0; --l) {
         const uint64_t row = I.par + (uint64_t)l * n;
-        sweep_bwd(x, y, lds, cur, g, A.mats, (int64_t)row, -1, true, redw);
+        build_tables(zt, g, A.mats + 4 * row, false);
+        sweep_bwd(x, y, lds, cur, g, A.mats, (int64_t)row, -1, zt, one, false, false, 0, 0, redw);
         __syncthreads();
         for (int q = tid; q < n; q += NT) {
-          double s[3] = {0, 0, 0};
-          for (int w = 0; w < NW; ++w)
-            for (int kk = 0; kk < 3; ++kk) s[kk] += red[w * SLOTS + 3 * n + 3 * q + kk];
-          const double ph = A.ang[2 * (row + q) + 1];
-          A.grad[2 * (row + q)] = cos(ph) * s[1] - sin(ph) * s[0];
-          A.grad[2 * (row + q) + 1] = s[2];
+          double st = 0, sp = 0;
+          for (int w = 0; w < NW; ++w) {
+            st += red[w * SLOTS + M + q];
+            sp += red[w * SLOTS + 2 * M + q];
+          }
+          A.grad[2 * (row + q)] = st;
+          A.grad[2 * (row + q) + 1] = sp;
         }
-        __syncthreads();
       }
     }
     if (A.flags & kKeep) {
@@ -422,8 +527,8 @@ struct Tile {
     }
   }
 
-  static __device__ void run_pass(const HeaArgs<T>& A, const HeaInst& I, int s, uint32_t t, C* lds, double* red,
-                                  double* redc)
+  static __device__ void run_pass(const HeaArgs<T>& A, const HeaInst& I, int s, uint32_t t, C* lds, C* zt,
+                                  double* red, double* redc)
   {
     const uint32_t tid = threadIdx.x;
     const int n = I.n, D = I.depth;
@@ -435,7 +540,8 @@ struct Tile {
     g.init(n, isB ? 2 : 1, isB ? c : 0, M, c, t);
     const int64_t row_pre = j >= 1 ? (int64_t)(I.par + (uint64_t)(j - 1) * n) : -1;
     const int64_t row_post = j < D ? (int64_t)(I.par + (uint64_t)j * n) : -1;
-    const bool cz = j >= 1;
+    // RZs deferred from pass j-1: the other tile kind's qubits (A: 0..M-1, B: M..n-1)
+    const int q0 = j >= 1 ? (isB ? 0 : M) : 0, q1 = j >= 1 ? (isB ? M : n) : 0;
     constexpr int L = M - LR;  // load/store layout: consecutive threads, consecutive amplitudes
     C* psi = A.arena + I.psi;
     C* lam = A.arena + I.lam;
@@ -454,14 +560,21 @@ struct Tile {
 #pragma unroll
       for (int k = 0; k < R; ++k) y[k] = lam[g.gidx(lidx<LR>(tid, k, L))];
     }
+    C f0{1, 0};
+    if (row_pre >= 0) {
+      build_tables(zt, g, A.mats + 4 * row_pre, isB);
+      f0 = deferred_factor(g, A.mats + 4 * row_pre, q0, q1);
+    }
+    const bool has_f0 = q1 > q0;
     int cur = L;
     double* redw = red + (tid >> 6) * SLOTS;
-    if (s <= D) sweep_fwd(x, lds, cur, g, A.mats, row_pre, row_post, cz);
+    if (s <= D) sweep_fwd(x, lds, cur, g, A.mats, row_pre, row_post, zt, f0, has_f0);
     if (s == D) {
       const double cs = cost_and_lam(x, y, grad, cur, g, redc);
       if (tid == 0) A.cpart[I.cpart + t] = cs;
     }
-    if (grad && s >= D) sweep_bwd(x, y, lds, cur, g, A.mats, row_pre, row_post, cz, redw);
+    if (grad && s >= D)
+      sweep_bwd(x, y, lds, cur, g, A.mats, row_pre, row_post, zt, f0, has_f0, isB, q0, q1, redw);
     const bool store_x = s < D || (s == D && (grad || (A.flags & kKeep))) || (s > D && s < 2 * D);
     const bool store_y = grad && s >= D && s < 2 * D;
     if ((store_x || store_y) && cur != L) {
@@ -479,16 +592,28 @@ struct Tile {
     }
     if (grad && s >= D) {
       __syncthreads();
+      // slots: [0, gg) dtheta of layer j (post), [M, M + gg) dtheta of layer j-1 (pre),
+      // [2M, 2M + n) dphi of layer j-1 (every qubit: pre bits and deferred)
       const int gg = g.lb - g.la;
-      for (int sl = tid; sl < 6 * gg; sl += NT) {
-        const int which = sl / (3 * gg);  // 0: layer j (post), 1: layer j-1 (pre)
-        const int layer = which ? j - 1 : j;
+      const int nsl = 2 * gg + (row_pre >= 0 ? n : 0);
+      for (int sl = tid; sl < nsl; sl += NT) {
+        int layer, q, kind, slot;
+        if (sl < 2 * gg) {
+          const int which = sl / gg;  // 0: layer j (post), 1: layer j-1 (pre)
+          layer = which ? j - 1 : j;
+          q = g.gq(g.la + sl % gg);
+          kind = 0;
+          slot = which * M + sl % gg;
+        } else {
+          layer = j - 1;
+          q = sl - 2 * gg;
+          kind = 1;
+          slot = 2 * M + q;
+        }
         if (layer < 0 || layer >= D) continue;
         double v = 0;
-        for (int w = 0; w < NW; ++w) v += red[w * SLOTS + sl];
-        const int r = sl - which * 3 * gg;
-        const int q = g.gq(g.la + r / 3);
-        A.part[I.part + ((((uint64_t)layer * n + q) * 3 + r % 3) << I.ntl) + t] = v;
+        for (int w = 0; w < NW; ++w) v += red[w * SLOTS + slot];
+        A.part[I.part + ((((uint64_t)layer * n + q) * 2 + kind) << I.ntl) + t] = v;
       }
     }
   }
@@ -499,6 +624,7 @@ __global__ __launch_bounds__((Tile<T, M, R>::NT), 2) void k_hea(HeaArgs<T> A)
 {
   using Tl = Tile<T, M, R>;
   __shared__ V<T> lds[1 << M];
+  __shared__ V<T> zt[Tl::KMAX * R];
   __shared__ double red[Tl::NW * Tl::SLOTS];
   __shared__ double redc[Tl::NW];
   __shared__ uint32_t sh[2];
@@ -534,7 +660,7 @@ __global__ __launch_bounds__((Tile<T, M, R>::NT), 2) void k_hea(HeaArgs<T> A)
     const HeaInst I = sh_in;
     __syncthreads();  // every thread holds its copy before thread 0 stages the next task
     if (I.whole) {
-      Tl::run_whole(A, I, tk.inst, lds, red, redc);
+      Tl::run_whole(A, I, tk.inst, lds, zt, red, redc);
       if (tid == 0) {
         stage(next);
         next = atomicAdd(head, 1u);
@@ -562,7 +688,7 @@ __global__ __launch_bounds__((Tile<T, M, R>::NT), 2) void k_hea(HeaArgs<T> A)
       const bool ok = sh[1];
       if (ok) {
         if (tk.step > 0) acquire_tiles();
-        Tl::run_pass(A, I, tk.step, tk.tile, lds, red, redc);
+        Tl::run_pass(A, I, tk.step, tk.tile, lds, zt, red, redc);
       }
       __syncthreads();  // sh[1] read by all before it is reused
       if (tid == 0) {
@@ -602,7 +728,8 @@ __global__ void k_hea_zero(uint32_t* p, uint32_t n)
   if (i < n) p[i] = 0;
 }
 
-// U = RZ(phi) RY(theta) per pair, computed in FP64 and rounded once to T.
+// Per (layer, qubit) pair: (cos theta/2, sin theta/2) and (cos phi/2, sin phi/2), computed in
+// FP64 and rounded once to T (RY = [[c, -s], [s, c]], RZ = diag(e^{-i phi/2}, e^{i phi/2})).
 template <typename T>
 __global__ void k_hea_mats(const double* __restrict__ ang, V<T>* __restrict__ mats, uint64_t npairs)
 {
@@ -611,16 +738,16 @@ __global__ void k_hea_mats(const double* __restrict__ ang, V<T>* __restrict__ ma
   double s, c, sp, cp;
   sincos(0.5 * ang[2 * p], &s, &c);
   sincos(0.5 * ang[2 * p + 1], &sp, &cp);
-  mats[4 * p + 0] = V<T>{(T)(cp * c), (T)(-sp * c)};
-  mats[4 * p + 1] = V<T>{(T)(-cp * s), (T)(sp * s)};
-  mats[4 * p + 2] = V<T>{(T)(cp * s), (T)(sp * s)};
-  mats[4 * p + 3] = V<T>{(T)(cp * c), (T)(sp * c)};
+  mats[4 * p + 0] = V<T>{(T)c, (T)s};
+  mats[4 * p + 1] = V<T>{(T)cp, (T)sp};
+  mats[4 * p + 2] = V<T>{0, 0};
+  mats[4 * p + 3] = V<T>{0, 0};
 }
 
 // One block per (tiled instance, layer): thread q sums the tile partials of qubit q in
-// tile order and combines them into (d/dtheta, d/dphi); the costs are summed likewise.
+// tile order into (d/dtheta, d/dphi); the costs are summed likewise.
 __global__ void k_hea_epi(const HeaInst* __restrict__ inst, const uint2* __restrict__ items,
-                          const double* __restrict__ ang, const double* __restrict__ part,
+                          const double* __restrict__ part,
                           const double* __restrict__ cpart, double* grad, double* cost, int with_grad)
 {
   const uint2 it = items[blockIdx.x];
@@ -629,16 +756,14 @@ __global__ void k_hea_epi(const HeaInst* __restrict__ inst, const uint2* __restr
   const uint64_t T = uint64_t(1) << I.ntl;
   if (with_grad && q < I.n) {
     const uint64_t pi = I.par + (uint64_t)it.y * I.n + q;
-    const double* p = part + I.part + ((((uint64_t)it.y * I.n + q) * 3) << I.ntl);
-    double sx = 0, sy = 0, sz = 0;
+    const double* p = part + I.part + ((((uint64_t)it.y * I.n + q) * 2) << I.ntl);
+    double st = 0, sp = 0;
     for (uint64_t t = 0; t < T; ++t) {
-      sx += p[t];
-      sy += p[T + t];
-      sz += p[2 * T + t];
+      st += p[t];
+      sp += p[T + t];
     }
-    const double ph = ang[2 * pi + 1];
-    grad[2 * pi] = cos(ph) * sy - sin(ph) * sx;
-    grad[2 * pi + 1] = sz;
+    grad[2 * pi] = st;
+    grad[2 * pi + 1] = sp;
   }
   if (it.y == 0 && q == blockDim.x - 1) {
     double v = 0;
@@ -798,7 +923,7 @@ int build(iqsh_hea* h, int mode)
       return fail(IQSH_ERR_NOTSUP, "instance %d: %d qubits exceed twice the tile (%d qubits)", b, h->n[b], v.M);
     x.ntl = h->n[b] - v.M;
     x.part = part;
-    part += (uint64_t)h->d[b] * h->n[b] * 3 << x.ntl;
+    part += (uint64_t)h->d[b] * h->n[b] * 2 << x.ntl;
     x.cpart = cpart;
     cpart += uint64_t(1) << x.ntl;
     x.arrive = arrive;
@@ -954,7 +1079,7 @@ int enqueue_t(iqsh_hea* h, int mode, hipStream_t st)
   IQSI_TRY(launch_var<T>(h->tune.variant, a, h->tune.workers_per_cu, h->cus, st));
   if (h->nepi) {
     hipLaunchKernelGGL(k_hea_epi, dim3(h->nepi), dim3(64), 0, st, (const HeaInst*)h->inst, (const uint2*)h->epi,
-                       (const double*)h->ang, (const double*)h->part, (const double*)h->cpart, h->grad, h->cost,
+                       (const double*)h->part, (const double*)h->cpart, h->grad, h->cost,
                        (int)(mode & kGrad));
     IQSI_TRY(hipGetLastError());
   }

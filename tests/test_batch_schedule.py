@@ -1,12 +1,17 @@
 """The tile-pass schedule of the batched HEA kernel (intel-qs_amd/csrc/iqs_batch.hip),
-restated on the CPU: pass j gates its locality with R_{j-1}, the CZ ladder of layer j-1 and
-R_j, in register phases of log2(R) consecutive local bits.  Checked here:
-  * every rotation is applied exactly once and every CZ pair once per layer, after the two
-    rotations of its layer and before the two of the next (all sizes up to 26 qubits);
+restated on the CPU.  A layer U_l = CZ_l . prod_q RZ_q(phi_lq) . prod_q RY_q(theta_lq) is
+split into real RY rotations and one diagonal per register phase: pass j, in register phases
+of log2(R) consecutive local bits, applies RY_{j-1} on the bits of its locality, a diagonal
+(RZ_{j-1} of those bits, CZ pairs of layer j-1, and in phase 0 the RZ_{j-1} of the qubits
+the previous pass rotated: the deferred RZs), then RY_j.  Checked here:
+  * every RY and every RZ is applied exactly once and every CZ pair once per layer; each RZ
+    sits after its qubit's RY of the same layer and before the RY of the next; each CZ pair
+    after the two RYs of its layer and before the two of the next (all sizes up to 26
+    qubits, every tile shape);
   * the register ranges of every phase hold the bits they gate;
   * executing the schedule on a state vector gives the circuit's state, and the adjoint
-    formulas (dC/dphi = Im<l|Z|p>, dC/dtheta = cos(phi) Im<l|Y|p> - sin(phi) Im<l|X|p>)
-    match central finite differences of the plain circuit.
+    formulas (dC/dtheta = Im<l|Y|p> just after the RY, dC/dphi = Im<l|Z|p> at the diagonal
+    holding the RZ) match central finite differences of the plain circuit.
 The arithmetic mirrors the kernel's Geo struct (phase bounds, local->global maps, CZ masks)."""
 import itertools
 import math
@@ -45,20 +50,24 @@ class Geo:
             m |= self.rng(self.gq(self.lb - 1), self.n - 1)
         return m & self.rng(0, self.n - 1)
 
+    def gated(self):
+        return [self.gq(b) for b in range(self.la, self.lb)]
 
-def ops_fwd(g, pre, post, cz):
+
+def ops_fwd(g, pre, post, cz, deferred):
+    """One pass: ("RY", layer, q) and ("D", layer, rz-qubits, cz-mask) in execution order."""
     ops = []
     for k in range(g.K):
         nb, ne, a = g.nb(k), g.ne(k), g.a(k)
         assert 0 <= a <= nb and ne <= a + g.LR <= g.M
         if pre is not None:
-            ops += [("R", pre, g.gq(b)) for b in range(nb, ne)]
-        if cz:
-            ops.append(("CZ", g.czmask(k)))
+            ops += [("RY", pre, g.gq(b)) for b in range(nb, ne)]
+            zs = [g.gq(b) for b in range(nb, ne)] + (list(deferred) if k == 0 else [])
+            ops.append(("D", pre, zs, g.czmask(k) if cz else 0))
         if post is not None:
             lo, hi = max(nb - 1, g.la), (g.lb if k == g.K - 1 else ne - 1)
             assert lo >= a and hi <= a + g.LR
-            ops += [("R", post, g.gq(b)) for b in range(lo, hi)]
+            ops += [("RY", post, g.gq(b)) for b in range(lo, hi)]
     return ops
 
 
@@ -66,13 +75,14 @@ def schedule(n, D, M, LR):
     """Forward gate sequence of one register as the kernel executes it."""
     if n <= M:
         g = Geo(M, LR, n, 0, 0, n, 0)
-        return [op for l in range(D) for op in ops_fwd(g, l, None, True)]
+        return [op for l in range(D) for op in ops_fwd(g, l, None, True, [])]
     c = 2 * M - n
-    seq = []
+    seq, prev = [], []
     for j in range(D + 1):
         B = j & 1
         g = Geo(M, LR, n, 2 if B else 1, c if B else 0, M, c)
-        seq += ops_fwd(g, j - 1 if j >= 1 else None, j if j < D else None, j >= 1)
+        seq += ops_fwd(g, j - 1 if j >= 1 else None, j if j < D else None, j >= 1, prev)
+        prev = g.gated()  # this pass's RY_j leave their RZ_j to the next pass
     return seq
 
 
@@ -84,31 +94,43 @@ def test_schedule_covers_circuit_in_order(M, LR):
     for n in range(1, min(2 * M, 26) + 1):
         for D in (1, 2, 3, 5):
             seq = schedule(n, D, M, LR)
-            pos = {}
+            ry, rz = {}, {}
             for i, op in enumerate(seq):
-                if op[0] == "R":
-                    assert (op[1], op[2]) not in pos
-                    pos[(op[1], op[2])] = i
-            assert len(pos) == n * D
+                if op[0] == "RY":
+                    assert (op[1], op[2]) not in ry
+                    ry[(op[1], op[2])] = i
+                else:
+                    for q in op[2]:
+                        assert (op[1], q) not in rz
+                        rz[(op[1], q)] = i
+            assert len(ry) == n * D and len(rz) == n * D
+            for (l, q), i in rz.items():
+                assert ry[(l, q)] < i
+                if l + 1 < D:
+                    assert i < ry[(l + 1, q)]
             cnt = [0] * max(n - 1, 1)
             for i, op in enumerate(seq):
-                if op[0] != "CZ":
+                if op[0] != "D":
                     continue
                 for p in range(n - 1):
-                    if (op[1] >> p) & 1:
-                        l = cnt[p]
+                    if (op[3] >> p) & 1:
+                        assert cnt[p] == op[1]
                         cnt[p] += 1
                         for q in (p, p + 1):
-                            assert pos[(l, q)] < i
-                            if l + 1 < D:
-                                assert i < pos[(l + 1, q)]
+                            assert ry[(op[1], q)] < i
+                            if op[1] + 1 < D:
+                                assert i < ry[(op[1] + 1, q)]
             assert all(x == D for x in cnt[: n - 1])
 
 
-def u_mat(th, ph):
+def ry_mat(th):
     c, s = math.cos(th / 2), math.sin(th / 2)
+    return np.array([[c, -s], [s, c]], dtype=complex)
+
+
+def u_mat(th, ph):
     e = np.exp(-0.5j * ph)
-    return np.array([[e * c, -e * s], [np.conj(e) * s, np.conj(e) * c]])
+    return np.diag([e, np.conj(e)]) @ ry_mat(th)
 
 
 def apply1(psi, q, m):
@@ -126,6 +148,15 @@ def czd(n, mask):
     return np.where(par == 1, -1.0, 1.0)
 
 
+def diag_of(n, zs, czm, ph_row):
+    """The diagonal of op ("D", l, zs, czm): RZ(phi_lq) on q in zs, then the CZ pairs."""
+    x = np.arange(1 << n)
+    d = czd(n, czm).astype(complex)
+    for q in zs:
+        d = d * np.exp(0.5j * ph_row[q] * (2 * ((x >> q) & 1) - 1))
+    return d
+
+
 def plain_cost(n, th, ph):
     psi = np.zeros(1 << n, complex)
     psi[0] = 1
@@ -141,23 +172,27 @@ def run_schedule(n, seq, th, ph):
     psi = np.zeros(1 << n, complex)
     psi[0] = 1
     for op in seq:
-        psi = apply1(psi, op[2], u_mat(th[op[1], op[2]], ph[op[1], op[2]])) if op[0] == "R" else psi * czd(n, op[1])
+        if op[0] == "RY":
+            psi = apply1(psi, op[2], ry_mat(th[op[1], op[2]]))
+        else:
+            psi = psi * diag_of(n, op[2], op[3], ph[op[1]])
     o = n - 2 * np.array([bin(v).count("1") for v in range(1 << n)])
     cost = float(np.sum(o * np.abs(psi) ** 2))
     lam = o * psi
     g = np.zeros(th.shape + (2,))
-    X = np.array([[0, 1], [1, 0]])
     Y = np.array([[0, -1j], [1j, 0]])
-    Z = np.diag([1, -1])
+    x = np.arange(1 << n)
     for op in reversed(seq):
-        if op[0] == "R":
+        if op[0] == "RY":
             l, q = op[1], op[2]
-            sx, sy, sz = (np.imag(np.vdot(lam, apply1(psi, q, P))) for P in (X, Y, Z))
-            g[l, q] = (math.cos(ph[l, q]) * sy - math.sin(ph[l, q]) * sx, sz)
-            ud = u_mat(th[l, q], ph[l, q]).conj().T
-            psi, lam = apply1(psi, q, ud), apply1(lam, q, ud)
+            g[l, q, 0] = np.imag(np.vdot(lam, apply1(psi, q, Y)))
+            rd = ry_mat(th[l, q]).T  # RY^dagger (real)
+            psi, lam = apply1(psi, q, rd), apply1(lam, q, rd)
         else:
-            d = czd(n, op[1])
+            t = np.imag(np.conj(lam) * psi)
+            for q in op[2]:
+                g[op[1], q, 1] = float(np.sum(t * (1 - 2 * ((x >> q) & 1))))  # Im<l|Z_q|p>
+            d = np.conj(diag_of(n, op[2], op[3], ph[op[1]]))
             psi, lam = psi * d, lam * d
     return cost, g
 

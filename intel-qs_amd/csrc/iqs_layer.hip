@@ -3,12 +3,13 @@
 // include/iqs_hip.h ("layers").
 //
 // Instead of n passes over the state (one per qubit), a layer takes
-//   * one lane-shuffle pass for positions 0..5: each lane holds one amplitude, partners at
-//     lane ^ 2^q come through __shfl_xor, every load/store is a contiguous 1 KiB run;
+//   * one lane pass for positions 0..7: each lane holds 4 amplitudes 64 apart, partners at
+//     lane ^ 2^q (positions 0-5) come through __shfl_xor, positions 6 and 7 pair the
+//     lane's own registers, and every load/store is a contiguous run of 64 amplitudes;
 //   * passes of up to four positions each above that: a thread holds the 2^m amplitudes of
 //     one group in registers (consecutive threads take consecutive groups, so every
 //     instruction is a contiguous run) and applies the m rotations there.
-// At 28 qubits: 7 passes instead of 28.
+// At 28 qubits: 6 passes instead of 28.
 //
 // Adjoint: the gates of a layer act on different qubits, so each generator G_q commutes
 // with every gate of the layer except its own, and inserting G_q after U_q equals inserting
@@ -32,6 +33,7 @@ namespace {
 constexpr int kLB = 256;          // threads per workgroup
 constexpr int kLowQ = 6;          // positions handled by the lane-shuffle pass
 constexpr int kLowP = 4;          // amplitudes per lane in the lane-shuffle pass
+constexpr int kQaoaAdjP = 4;      // amplitudes of psi and of lambda per lane in the QAOA adjoint lane pass
 constexpr int kRegQ = 4;          // positions per register pass
 constexpr unsigned kMaxGrid = 2048;    // workgroups of the grid-stride register passes (small states)
 // Lane passes: kLowRows rows of kLB * kLowP amplitudes per workgroup (28-qubit complex128:
@@ -119,10 +121,46 @@ __device__ __forceinline__ V shfl_v(const V& a, int mask)
   return r;
 }
 
-// Lane-shuffle pass over positions 0..L-1 (L <= 6).  N a multiple of kLB * kLowP.
+// Row layout of the lane passes: a row of kLB * P amplitudes, thread t holding in-row
+// indices (wave << (6 + log2 P)) | (j << 6) | lane for j < P.  Positions 0-5 are lane bits
+// (exchanged by shuffles), positions 6 .. 5 + log2 P register bits (register-local pairs),
+// and every load/store instruction covers 64 consecutive amplitudes.
+template <int P>
+__device__ __forceinline__ unsigned row_index(unsigned tid, int j)
+{
+  constexpr int LP = P == 1 ? 0 : (P == 2 ? 1 : (P == 4 ? 2 : 3));
+  static_assert((1 << LP) == P, "P must be 1, 2, 4 or 8");
+  return ((tid >> 6) << (6 + LP)) | ((unsigned)j << 6) | (tid & 63);
+}
+
+// The gate on the register positions of a lane pass (bit b of j: position 6 + b), with the
+// generator terms of the adjoint taken before each position is undone.
+template <typename T, bool ADJ, int P, typename V>
+__device__ __forceinline__ void reg_positions(V* x, V* y, const Mat2<T>& u, const Mat2<double>& g, double& acc)
+{
+#pragma unroll
+  for (int b = 0; (1 << b) < P; ++b)
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+      if ((j >> b) & 1) continue;
+      const int j1 = j | (1 << b);
+      const V a = x[j], c = x[j1];
+      if (ADJ) acc += im_term(y[j], g.m00, a, g.m01, c) + im_term(y[j1], g.m10, a, g.m11, c);
+      x[j] = mat_row(u.m00, a, u.m01, c);
+      x[j1] = mat_row(u.m10, a, u.m11, c);
+      if (ADJ) {
+        const V ya = y[j], yc = y[j1];
+        y[j] = mat_row(u.m00, ya, u.m01, yc);
+        y[j1] = mat_row(u.m10, ya, u.m11, yc);
+      }
+    }
+}
+
+// Lane pass over positions 0 .. kLowQ + log2(kLowP) - 1 (shuffles for 0-5, registers above).
+// N a multiple of kLB * kLowP.
 template <typename T, bool ADJ>
 __global__ __launch_bounds__(kLB) void k_layer_low(typename Vec<T>::t* __restrict__ psi, typename Vec<T>::t* __restrict__ lam,
-                                                   uint64_t N, int L, Mat2<T> u, Mat2<double> g, double* __restrict__ part)
+                                                   uint64_t N, Mat2<T> u, Mat2<double> g, double* __restrict__ part)
 {
   using V = typename Vec<T>::t;
   const int lane = threadIdx.x & 63;
@@ -134,14 +172,15 @@ __global__ __launch_bounds__(kLB) void k_layer_low(typename Vec<T>::t* __restric
   const uint64_t chunk = (rows + G - 1) / G;
   const uint64_t b0 = lb * chunk, b1 = (b0 + chunk < rows) ? b0 + chunk : rows;
   for (uint64_t rb = b0; rb < b1; ++rb) {
-    const uint64_t r0 = rb * (kLB * kLowP) + threadIdx.x;
+    const uint64_t r0 = rb * (kLB * kLowP);
     V x[kLowP], y[kLowP];
 #pragma unroll
     for (int j = 0; j < kLowP; ++j) {
-      x[j] = ldg(&psi[r0 + (uint64_t)j * kLB]);
-      if (ADJ) y[j] = ldg(&lam[r0 + (uint64_t)j * kLB]);
+      x[j] = ldg(&psi[r0 + row_index<kLowP>(threadIdx.x, j)]);
+      if (ADJ) y[j] = ldg(&lam[r0 + row_index<kLowP>(threadIdx.x, j)]);
     }
-    for (int q = 0; q < L; ++q) {
+#pragma unroll 1
+    for (int q = 0; q < kLowQ; ++q) {
       const int msk = 1 << q;
       const bool hi = (lane >> q) & 1;
 #pragma unroll
@@ -157,10 +196,11 @@ __global__ __launch_bounds__(kLB) void k_layer_low(typename Vec<T>::t* __restric
         x[j] = hi ? mat_row(u.m10, x0, u.m11, x1) : mat_row(u.m00, x0, u.m01, x1);
       }
     }
+    reg_positions<T, ADJ, kLowP>(x, y, u, g, acc);
 #pragma unroll
     for (int j = 0; j < kLowP; ++j) {
-      stg(x[j], &psi[r0 + (uint64_t)j * kLB]);
-      if (ADJ) stg(y[j], &lam[r0 + (uint64_t)j * kLB]);
+      stg(x[j], &psi[r0 + row_index<kLowP>(threadIdx.x, j)]);
+      if (ADJ) stg(y[j], &lam[r0 + row_index<kLowP>(threadIdx.x, j)]);
     }
   }
   if (ADJ) wg_sum_store(acc, part);
@@ -180,14 +220,12 @@ __global__ __launch_bounds__(kLB) void k_layer_low(typename Vec<T>::t* __restric
 // plain lane pass.  W: weighted cuts, summed from byte tables of weight sums in LDS.
 template <typename T, bool ADJ, bool W>
 __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict__ psi, typename Vec<T>::t* __restrict__ lam,
-                                                  uint64_t N, int L, Mat2<T> u, Mat2<double> g, CutSpec cs,
+                                                  uint64_t N, Mat2<T> u, Mat2<double> g, CutSpec cs,
                                                   double* __restrict__ part)
 {
   using V = typename Vec<T>::t;
-  // The adjoint holds psi and lambda: 2 amplitudes of each per lane keep it at 4 waves per
-  // SIMD instead of 2.
-  constexpr int P = ADJ ? 2 : kLowP;
-  constexpr int RB = ADJ ? 9 : 10;  // log2 of a row's amplitudes
+  constexpr int P = ADJ ? kQaoaAdjP : kLowP;
+  constexpr int RB = 8 + (P == 1 ? 0 : (P == 2 ? 1 : (P == 4 ? 2 : 3)));  // log2 of a row's amplitudes
   static_assert(kLB * P == (1 << RB), "row size");
   __shared__ uint64_t tab[W ? 8 : 1][W ? 256 : 1];
   __shared__ double wt[W ? 8 : 1][W ? 256 : 1];
@@ -200,7 +238,7 @@ __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict
     for (unsigned c = threadIdx.x; c <= cs.ne; c += kLB) ph[c] = Cx<T>{(T)cs.ph_re[c], (T)cs.ph_im[c]};
 #pragma unroll
     for (int j = 0; j < P; ++j) {
-      const unsigned xl = threadIdx.x + j * kLB;
+      const unsigned xl = row_index<P>(threadIdx.x, j);
       uint64_t e = 0;
 #pragma unroll
       for (int b = 0; b < RB; ++b)
@@ -216,7 +254,7 @@ __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict
   const uint64_t chunk = (rows + G - 1) / G;
   const uint64_t b0 = lb * chunk, b1 = (b0 + chunk < rows) ? b0 + chunk : rows;
   for (uint64_t rb = b0; rb < b1; ++rb) {
-    const uint64_t r0 = (rb << RB) + threadIdx.x;
+    const uint64_t r0 = rb << RB;
     uint64_t ehi = 0;
     if constexpr (!W)
       for (uint64_t r = rb; r; r &= r - 1) ehi ^= cs.inc.c[RB + __builtin_ctzll(r)];
@@ -225,22 +263,23 @@ __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict
     double cut[P];
 #pragma unroll
     for (int j = 0; j < P; ++j) {
-      x[j] = ldg(&psi[r0 + (uint64_t)j * kLB]);
-      if (ADJ) y[j] = ldg(&lam[r0 + (uint64_t)j * kLB]);
+      x[j] = ldg(&psi[r0 + row_index<P>(threadIdx.x, j)]);
+      if (ADJ) y[j] = ldg(&lam[r0 + row_index<P>(threadIdx.x, j)]);
       if constexpr (!W) {
         const int ic = __popcll(ehi ^ elo[j]);
         cut[j] = (double)ic;
         f[j] = ph[ic];
       } else {
         int ic;
-        cut[j] = cut_value(tab, wt, cs, r0 + (uint64_t)j * kLB, &ic);
+        cut[j] = cut_value(tab, wt, cs, r0 + row_index<P>(threadIdx.x, j), &ic);
         double sn, co;
         sincos(cs.gamma * cut[j], &sn, &co);
         f[j] = Cx<T>{(T)co, (T)-sn};
       }
       if (!ADJ) x[j] = cmul(f[j], x[j]);
     }
-    for (int q = 0; q < L; ++q) {
+#pragma unroll 1
+    for (int q = 0; q < kLowQ; ++q) {
       const int msk = 1 << q;
       const bool hi = (lane >> q) & 1;
 #pragma unroll
@@ -256,15 +295,16 @@ __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict
         x[j] = hi ? mat_row(u.m10, x0, u.m11, x1) : mat_row(u.m00, x0, u.m01, x1);
       }
     }
+    reg_positions<T, ADJ, P>(x, y, u, g, acc);
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       if (ADJ) {
         accg += cut[j] * ((double)y[j].x * x[j].y - (double)y[j].y * x[j].x);
         x[j] = cmul(f[j], x[j]);
         y[j] = cmul(f[j], y[j]);
-        stg(y[j], &lam[r0 + (uint64_t)j * kLB]);
+        stg(y[j], &lam[r0 + row_index<P>(threadIdx.x, j)]);
       }
-      stg(x[j], &psi[r0 + (uint64_t)j * kLB]);
+      stg(x[j], &psi[r0 + row_index<P>(threadIdx.x, j)]);
     }
   }
   if (ADJ) {
@@ -427,24 +467,27 @@ int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm,
   if (ADJ) g = Mat2<double>{{gm[0], gm[1]}, {gm[2], gm[3]}, {gm[4], gm[5]}, {gm[6], gm[7]}};
   double acc = 0, accg = 0;
   const unsigned n = h->n;
-  const int L = (int)std::min<unsigned>(n, kLowQ);
+  // Amplitudes per lane of this lane pass and the positions it covers (0-5 by shuffles,
+  // log2 P more in registers).
+  const int lowp = (ADJ && cs) ? kQaoaAdjP : kLowP;
+  const unsigned lowq = kLowQ + (lowp == 1 ? 0 : (lowp == 2 ? 1 : (lowp == 4 ? 2 : 3)));
   // Forward: the lane pass first (it carries the phase layer, applied before the mixer);
   // adjoint: last (the phase is undone after the whole mixer).
   auto low_pass = [&]() -> int {
     // kLowRows rows per workgroup.  The weighted QAOA form builds byte tables of weight sums
     // per workgroup: a persistent grid of <= kMaxGrid workgroups for it.
-    const uint64_t rows = h->size() / (kLB * ((ADJ && cs) ? 2 : kLowP));
+    const uint64_t rows = h->size() / (kLB * (uint64_t)lowp);
     const unsigned grid = (cs && cs->weighted) ? (unsigned)std::min<uint64_t>(rows, kMaxGrid)
                                                : (unsigned)std::min<uint64_t>(std::max<uint64_t>(1, rows / kLowRows), kLowMaxGrid);
     if (ADJ)
       if (int rc = part_buf(2 * (uint64_t)grid + 2)) return rc;
     V* lp = ADJ ? (V*)lam->d : nullptr;
     if (cs && cs->weighted)
-      k_qaoa_low<T, ADJ, true><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, h->size(), L, u, g, *cs, g_lw.part);
+      k_qaoa_low<T, ADJ, true><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, h->size(), u, g, *cs, g_lw.part);
     else if (cs)
-      k_qaoa_low<T, ADJ, false><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, h->size(), L, u, g, *cs, g_lw.part);
+      k_qaoa_low<T, ADJ, false><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, h->size(), u, g, *cs, g_lw.part);
     else
-      k_layer_low<T, ADJ><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, ADJ ? (V*)lam->d : nullptr, h->size(), L, u, g,
+      k_layer_low<T, ADJ><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, ADJ ? (V*)lam->d : nullptr, h->size(), u, g,
                                                              g_lw.part);
     IQSI_TRY(hipGetLastError());
     if (!ADJ) return IQSH_OK;
@@ -455,10 +498,11 @@ int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm,
   if (!ADJ)
     if (int rc = low_pass()) return rc;
   // Positions above the lane pass in ceil(rest / 4) passes of as equal a size as possible
-  // (28 qubits: 4, 4, 4, 4, 3, 3 -- a pass of two positions streams ~20% slower).
-  const unsigned rest = n > (unsigned)kLowQ ? n - kLowQ : 0;
+  // (28 qubits, lane pass over 0-7: 4, 4, 4, 4, 4 -- a pass of two positions streams ~20%
+  // slower).
+  const unsigned rest = n > lowq ? n - lowq : 0;
   const unsigned npass = (rest + kRegQ - 1) / kRegQ;
-  for (unsigned i = 0, p = kLowQ; i < npass; ++i) {
+  for (unsigned i = 0, p = lowq; i < npass; ++i) {
     const unsigned m = rest / npass + (i < rest % npass ? 1 : 0);
     int rc = IQSH_OK;
     if (m == 4) rc = reg_pass<T, ADJ, 4>(h, lam, p, u, g, &acc);
@@ -477,6 +521,8 @@ int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm,
 
 // The lane pass needs whole rows of kLB * kLowP amplitudes with lanes spanning >= 64.
 constexpr unsigned kMinLayerQubits = 10;
+static_assert(kMinLayerQubits >= 8 + 2 && kLB * kLowP <= (1 << kMinLayerQubits) && kLB * kQaoaAdjP <= (1 << kMinLayerQubits),
+              "a lane pass row must fit the smallest register");
 
 }  // namespace
 

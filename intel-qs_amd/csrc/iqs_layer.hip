@@ -34,7 +34,8 @@ constexpr int kLB = 256;          // threads per workgroup
 constexpr int kLowQ = 6;          // positions handled by the lane-shuffle pass
 constexpr int kLowP = 4;          // amplitudes per lane in the lane-shuffle pass
 constexpr int kQaoaAdjP = 4;      // amplitudes of psi and of lambda per lane in the QAOA adjoint lane pass
-constexpr int kRegQ = 4;          // positions per register pass
+constexpr int kRegQ = 4;          // positions per register pass (adjoint: psi and lambda)
+constexpr int kRegQF = 5;         // positions per register pass (forward)
 constexpr unsigned kMaxGrid = 2048;    // workgroups of the grid-stride register passes (small states)
 // Lane passes: kLowRows rows of kLB * kLowP amplitudes per workgroup (28-qubit complex128:
 // forward 4.84 -> 5.45 TB/s, adjoint 4.07 -> 4.35 against 2048 workgroups with 128 rows each;
@@ -501,11 +502,13 @@ int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm,
   // (28 qubits, lane pass over 0-7: 4, 4, 4, 4, 4 -- a pass of two positions streams ~20%
   // slower).
   const unsigned rest = n > lowq ? n - lowq : 0;
-  const unsigned npass = (rest + kRegQ - 1) / kRegQ;
+  const unsigned perq = ADJ ? kRegQ : kRegQF;
+  const unsigned npass = (rest + perq - 1) / perq;
   for (unsigned i = 0, p = lowq; i < npass; ++i) {
     const unsigned m = rest / npass + (i < rest % npass ? 1 : 0);
     int rc = IQSH_OK;
-    if (m == 4) rc = reg_pass<T, ADJ, 4>(h, lam, p, u, g, &acc);
+    if (m == 5) rc = reg_pass<T, false, 5>(h, lam, p, u, g, &acc);
+    else if (m == 4) rc = reg_pass<T, ADJ, 4>(h, lam, p, u, g, &acc);
     else if (m == 3) rc = reg_pass<T, ADJ, 3>(h, lam, p, u, g, &acc);
     else if (m == 2) rc = reg_pass<T, ADJ, 2>(h, lam, p, u, g, &acc);
     else rc = reg_pass<T, ADJ, 1>(h, lam, p, u, g, &acc);

@@ -1415,7 +1415,7 @@ int do_permute(iqsh_handle h, const unsigned* src_pos)
       return fail(IQSH_ERR_ALLOC, "permutation buffer: %s", hipGetErrorString(e));
     }
   }
-  // Tile bits: the L = perm_low<T>() lowest destination bits (512-byte runs) and every
+  // Tile bits: the L = perm_low<T>() lowest destination bits (runs of 32 amplitudes) and every
   // destination bit whose source position is below L.  If that adds bits (low positions
   // move up), the tiled transpose reads and writes whole runs; otherwise the gather's reads
   // are already runs.
@@ -1435,6 +1435,8 @@ int do_permute(iqsh_handle h, const unsigned* src_pos)
       ++m;
     }
   if (tiled) {
+    // k_permute_tiled moves exactly 2^(2L) amplitudes per workgroup (n >= 16 >= 2L).
+    if (m != 2 * L) return fail(IQSH_ERR_ARG, "permutation tile of %u bits (%u expected)", m, 2 * L);
     PermTile pt;
     std::memset(&pt, 0, sizeof pt);
     pt.m = m;

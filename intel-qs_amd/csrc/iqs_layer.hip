@@ -87,6 +87,19 @@ __device__ __forceinline__ double im_term(const V& y, const Cx<double>& a, const
   return (double)y.x * gi - (double)y.y * gr;
 }
 
+// Generator term of one amplitude of a pair (x0, x1): Im(conj(y) (G x)_row), row 1 if hi.
+// GX: G = X (the QAOA mixer's generator), (G x)_lo = x1 and (G x)_hi = x0 -- two products
+// instead of ten.
+template <bool GX, typename V>
+__device__ __forceinline__ double gen_term(const V& y, const Mat2<double>& g, const V& x0, const V& x1, bool hi)
+{
+  if (GX) {
+    const V o = hi ? x0 : x1;
+    return (double)y.x * (double)o.y - (double)y.y * (double)o.x;
+  }
+  return hi ? im_term(y, g.m10, x0, g.m11, x1) : im_term(y, g.m00, x0, g.m01, x1);
+}
+
 // Workgroup sum of v (fixed order: lanes by xor-shuffle, then the waves in order) stored
 // at out[slot] by thread 0.
 __device__ __forceinline__ void wg_sum_store(double v, double* out, uint64_t slot)
@@ -136,7 +149,7 @@ __device__ __forceinline__ unsigned row_index(unsigned tid, int j)
 
 // The gate on the register positions of a lane pass (bit b of j: position 6 + b), with the
 // generator terms of the adjoint taken before each position is undone.
-template <typename T, bool ADJ, int P, typename V>
+template <typename T, bool ADJ, int P, bool GX, typename V>
 __device__ __forceinline__ void reg_positions(V* x, V* y, const Mat2<T>& u, const Mat2<double>& g, double& acc)
 {
 #pragma unroll
@@ -146,7 +159,7 @@ __device__ __forceinline__ void reg_positions(V* x, V* y, const Mat2<T>& u, cons
       if ((j >> b) & 1) continue;
       const int j1 = j | (1 << b);
       const V a = x[j], c = x[j1];
-      if (ADJ) acc += im_term(y[j], g.m00, a, g.m01, c) + im_term(y[j1], g.m10, a, g.m11, c);
+      if (ADJ) acc += gen_term<GX>(y[j], g, a, c, false) + gen_term<GX>(y[j1], g, a, c, true);
       x[j] = mat_row(u.m00, a, u.m01, c);
       x[j1] = mat_row(u.m10, a, u.m11, c);
       if (ADJ) {
@@ -159,7 +172,7 @@ __device__ __forceinline__ void reg_positions(V* x, V* y, const Mat2<T>& u, cons
 
 // Lane pass over positions 0 .. kLowQ + log2(kLowP) - 1 (shuffles for 0-5, registers above).
 // N a multiple of kLB * kLowP.
-template <typename T, bool ADJ>
+template <typename T, bool ADJ, bool GX>
 __global__ __launch_bounds__(kLB) void k_layer_low(typename Vec<T>::t* __restrict__ psi, typename Vec<T>::t* __restrict__ lam,
                                                    uint64_t N, Mat2<T> u, Mat2<double> g, double* __restrict__ part)
 {
@@ -189,7 +202,7 @@ __global__ __launch_bounds__(kLB) void k_layer_low(typename Vec<T>::t* __restric
         const V xp = shfl_v(x[j], msk);
         const V x0 = hi ? xp : x[j], x1 = hi ? x[j] : xp;
         if (ADJ) {
-          acc += hi ? im_term(y[j], g.m10, x0, g.m11, x1) : im_term(y[j], g.m00, x0, g.m01, x1);
+          acc += gen_term<GX>(y[j], g, x0, x1, hi);
           const V yp = shfl_v(y[j], msk);
           const V y0 = hi ? yp : y[j], y1 = hi ? y[j] : yp;
           y[j] = hi ? mat_row(u.m10, y0, u.m11, y1) : mat_row(u.m00, y0, u.m01, y1);
@@ -197,7 +210,7 @@ __global__ __launch_bounds__(kLB) void k_layer_low(typename Vec<T>::t* __restric
         x[j] = hi ? mat_row(u.m10, x0, u.m11, x1) : mat_row(u.m00, x0, u.m01, x1);
       }
     }
-    reg_positions<T, ADJ, kLowP>(x, y, u, g, acc);
+    reg_positions<T, ADJ, kLowP, GX>(x, y, u, g, acc);
 #pragma unroll
     for (int j = 0; j < kLowP; ++j) {
       stg(x[j], &psi[r0 + row_index<kLowP>(threadIdx.x, j)]);
@@ -219,7 +232,7 @@ __global__ __launch_bounds__(kLB) void k_layer_low(typename Vec<T>::t* __restric
 // takes E(lo) of its P in-row offsets once, and E(row) comes from the row's set bits, the
 // same for the whole workgroup -- no per-workgroup tables, so short workgroups as in the
 // plain lane pass.  W: weighted cuts, summed from byte tables of weight sums in LDS.
-template <typename T, bool ADJ, bool W>
+template <typename T, bool ADJ, bool W, bool GX>
 __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict__ psi, typename Vec<T>::t* __restrict__ lam,
                                                   uint64_t N, Mat2<T> u, Mat2<double> g, CutSpec cs,
                                                   double* __restrict__ part)
@@ -259,25 +272,31 @@ __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict
     uint64_t ehi = 0;
     if constexpr (!W)
       for (uint64_t r = rb; r; r &= r - 1) ehi ^= cs.inc.c[RB + __builtin_ctzll(r)];
+    // Phase factor and cut of in-row amplitude j (forward: applied on load; adjoint: the
+    // phase term and the undo after the mixer, recomputed there so that they hold no
+    // registers across the shuffles).
+    auto phase_of = [&](int j, double* cut) -> Cx<T> {
+      if constexpr (!W) {
+        const int ic = __popcll(ehi ^ elo[j]);
+        *cut = (double)ic;
+        return ph[ic];
+      } else {
+        int ic;
+        *cut = cut_value(tab, wt, cs, r0 + row_index<P>(threadIdx.x, j), &ic);
+        double sn, co;
+        sincos(cs.gamma * *cut, &sn, &co);
+        return Cx<T>{(T)co, (T)-sn};
+      }
+    };
     V x[P], y[P];
-    Cx<T> f[P];
-    double cut[P];
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       x[j] = ldg(&psi[r0 + row_index<P>(threadIdx.x, j)]);
       if (ADJ) y[j] = ldg(&lam[r0 + row_index<P>(threadIdx.x, j)]);
-      if constexpr (!W) {
-        const int ic = __popcll(ehi ^ elo[j]);
-        cut[j] = (double)ic;
-        f[j] = ph[ic];
-      } else {
-        int ic;
-        cut[j] = cut_value(tab, wt, cs, r0 + row_index<P>(threadIdx.x, j), &ic);
-        double sn, co;
-        sincos(cs.gamma * cut[j], &sn, &co);
-        f[j] = Cx<T>{(T)co, (T)-sn};
+      if (!ADJ) {
+        double cut;
+        x[j] = cmul(phase_of(j, &cut), x[j]);
       }
-      if (!ADJ) x[j] = cmul(f[j], x[j]);
     }
 #pragma unroll 1
     for (int q = 0; q < kLowQ; ++q) {
@@ -288,7 +307,7 @@ __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict
         const V xp = shfl_v(x[j], msk);
         const V x0 = hi ? xp : x[j], x1 = hi ? x[j] : xp;
         if (ADJ) {
-          acc += hi ? im_term(y[j], g.m10, x0, g.m11, x1) : im_term(y[j], g.m00, x0, g.m01, x1);
+          acc += gen_term<GX>(y[j], g, x0, x1, hi);
           const V yp = shfl_v(y[j], msk);
           const V y0 = hi ? yp : y[j], y1 = hi ? y[j] : yp;
           y[j] = hi ? mat_row(u.m10, y0, u.m11, y1) : mat_row(u.m00, y0, u.m01, y1);
@@ -296,13 +315,15 @@ __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict
         x[j] = hi ? mat_row(u.m10, x0, u.m11, x1) : mat_row(u.m00, x0, u.m01, x1);
       }
     }
-    reg_positions<T, ADJ, P>(x, y, u, g, acc);
+    reg_positions<T, ADJ, P, GX>(x, y, u, g, acc);
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       if (ADJ) {
-        accg += cut[j] * ((double)y[j].x * x[j].y - (double)y[j].y * x[j].x);
-        x[j] = cmul(f[j], x[j]);
-        y[j] = cmul(f[j], y[j]);
+        double cut;
+        const Cx<T> f = phase_of(j, &cut);
+        accg += cut * ((double)y[j].x * x[j].y - (double)y[j].y * x[j].x);
+        x[j] = cmul(f, x[j]);
+        y[j] = cmul(f, y[j]);
         stg(y[j], &lam[r0 + row_index<P>(threadIdx.x, j)]);
       }
       stg(x[j], &psi[r0 + row_index<P>(threadIdx.x, j)]);
@@ -320,7 +341,7 @@ __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict
 // TILED: one group per thread, grid = items / kLB workgroups (items a power of two >=
 // kRegTileMin), one partial per workgroup at part[block_id()]; otherwise a grid-stride loop
 // over one contiguous chunk of groups per workgroup.
-template <typename T, bool ADJ, int M, bool TILED>
+template <typename T, bool ADJ, int M, bool TILED, bool GX>
 __global__ __launch_bounds__(kLB) void k_layer_reg(typename Vec<T>::t* __restrict__ psi, typename Vec<T>::t* __restrict__ lam,
                                                    uint64_t items, Ins<M> ins, Mat2<T> u, Mat2<double> g,
                                                    double* __restrict__ part)
@@ -367,7 +388,7 @@ __global__ __launch_bounds__(kLB) void k_layer_reg(typename Vec<T>::t* __restric
         for (int j = 0; j < D; ++j) {
           if ((j >> b) & 1) continue;
           const int j1 = j | (1 << b);
-          acc += im_term(y[j], g.m00, x[j], g.m01, x[j1]) + im_term(y[j1], g.m10, x[j], g.m11, x[j1]);
+          acc += gen_term<GX>(y[j], g, x[j], x[j1], false) + gen_term<GX>(y[j1], g, x[j], x[j1], true);
         }
     }
 #pragma unroll
@@ -427,7 +448,7 @@ int finish_pass(unsigned grid, double* acc)
   return IQSH_OK;
 }
 
-template <typename T, bool ADJ, int M>
+template <typename T, bool ADJ, int M, bool GX>
 int reg_pass(iqsh_handle h, iqsh_handle lam, unsigned p0, const Mat2<T>& u, const Mat2<double>& g, double* acc)
 {
   using V = typename Vec<T>::t;
@@ -443,14 +464,14 @@ int reg_pass(iqsh_handle h, iqsh_handle lam, unsigned p0, const Mat2<T>& u, cons
     if (ADJ)
       if (int rc = part_buf(G + 1)) return rc;
     const dim3 grid((unsigned)std::min<uint64_t>(G, 65536), (unsigned)((G + 65535) / 65536), 1);
-    k_layer_reg<T, ADJ, M, true><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, items, ins, u, g, g_lw.part);
+    k_layer_reg<T, ADJ, M, true, GX><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, items, ins, u, g, g_lw.part);
     IQSI_TRY(hipGetLastError());
     if (!ADJ) return IQSH_OK;
     double* out[1] = {acc};
     return device_sums(G, 1, out);
   }
   const unsigned grid = grid_for_work(items);
-  k_layer_reg<T, ADJ, M, false><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, items, ins, u, g, g_lw.part);
+  k_layer_reg<T, ADJ, M, false, GX><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, items, ins, u, g, g_lw.part);
   if (!ADJ) {
     IQSI_TRY(hipGetLastError());
     return IQSH_OK;
@@ -458,7 +479,7 @@ int reg_pass(iqsh_handle h, iqsh_handle lam, unsigned p0, const Mat2<T>& u, cons
   return finish_pass(grid, acc);
 }
 
-template <typename T, bool ADJ>
+template <typename T, bool ADJ, bool GX>
 int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm, const CutSpec* cs, double* grad,
              double* grad_phase)
 {
@@ -484,11 +505,11 @@ int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm,
       if (int rc = part_buf(2 * (uint64_t)grid + 2)) return rc;
     V* lp = ADJ ? (V*)lam->d : nullptr;
     if (cs && cs->weighted)
-      k_qaoa_low<T, ADJ, true><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, h->size(), u, g, *cs, g_lw.part);
+      k_qaoa_low<T, ADJ, true, GX><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, h->size(), u, g, *cs, g_lw.part);
     else if (cs)
-      k_qaoa_low<T, ADJ, false><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, h->size(), u, g, *cs, g_lw.part);
+      k_qaoa_low<T, ADJ, false, GX><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, h->size(), u, g, *cs, g_lw.part);
     else
-      k_layer_low<T, ADJ><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, ADJ ? (V*)lam->d : nullptr, h->size(), u, g,
+      k_layer_low<T, ADJ, GX><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, ADJ ? (V*)lam->d : nullptr, h->size(), u, g,
                                                              g_lw.part);
     IQSI_TRY(hipGetLastError());
     if (!ADJ) return IQSH_OK;
@@ -507,11 +528,11 @@ int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm,
   for (unsigned i = 0, p = lowq; i < npass; ++i) {
     const unsigned m = rest / npass + (i < rest % npass ? 1 : 0);
     int rc = IQSH_OK;
-    if (m == 5) rc = reg_pass<T, false, 5>(h, lam, p, u, g, &acc);
-    else if (m == 4) rc = reg_pass<T, ADJ, 4>(h, lam, p, u, g, &acc);
-    else if (m == 3) rc = reg_pass<T, ADJ, 3>(h, lam, p, u, g, &acc);
-    else if (m == 2) rc = reg_pass<T, ADJ, 2>(h, lam, p, u, g, &acc);
-    else rc = reg_pass<T, ADJ, 1>(h, lam, p, u, g, &acc);
+    if (m == 5) rc = reg_pass<T, false, 5, false>(h, lam, p, u, g, &acc);
+    else if (m == 4) rc = reg_pass<T, ADJ, 4, GX>(h, lam, p, u, g, &acc);
+    else if (m == 3) rc = reg_pass<T, ADJ, 3, GX>(h, lam, p, u, g, &acc);
+    else if (m == 2) rc = reg_pass<T, ADJ, 2, GX>(h, lam, p, u, g, &acc);
+    else rc = reg_pass<T, ADJ, 1, GX>(h, lam, p, u, g, &acc);
     if (rc) return rc;
     p += m;
   }
@@ -520,6 +541,19 @@ int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm,
   if (ADJ) *grad = 2.0 * acc;
   if (ADJ && grad_phase) *grad_phase = 2.0 * accg;
   return IQSH_OK;
+}
+
+// Backward step through a layer: the kernels with G = X when the generator is exactly X.
+int adjoint_layer(iqsh_handle psi, iqsh_handle lam, const double* uinv, const double* G, const CutSpec* cs,
+                  double* grad, double* grad_phase)
+{
+  static const double X[8] = {0, 0, 1, 0, 1, 0, 0, 0};
+  const bool gx = std::equal(G, G + 8, X);
+  if (psi->dtype == IQSH_C128)
+    return gx ? do_layer<double, true, true>(psi, lam, uinv, G, cs, grad, grad_phase)
+              : do_layer<double, true, false>(psi, lam, uinv, G, cs, grad, grad_phase);
+  return gx ? do_layer<float, true, true>(psi, lam, uinv, G, cs, grad, grad_phase)
+            : do_layer<float, true, false>(psi, lam, uinv, G, cs, grad, grad_phase);
 }
 
 // The lane pass needs whole rows of kLB * kLowP amplitudes with lanes spanning >= 64.
@@ -537,8 +571,8 @@ int iqsh_apply_1q_all(iqsh_handle h, const double m[8])
   if (!m) return iqsi::fail(IQSH_ERR_ARG, "null matrix");
   if (h->n < kMinLayerQubits) return iqsi::fail(IQSH_ERR_NOTSUP, "layer kernels need >= %u qubits", kMinLayerQubits);
   std::lock_guard<std::mutex> lk(g_lm);
-  return h->dtype == IQSH_C128 ? do_layer<double, false>(h, nullptr, m, nullptr, nullptr, nullptr, nullptr)
-                               : do_layer<float, false>(h, nullptr, m, nullptr, nullptr, nullptr, nullptr);
+  return h->dtype == IQSH_C128 ? do_layer<double, false, false>(h, nullptr, m, nullptr, nullptr, nullptr, nullptr)
+                               : do_layer<float, false, false>(h, nullptr, m, nullptr, nullptr, nullptr, nullptr);
 }
 
 int iqsh_qaoa_layer(iqsh_handle h, unsigned ne, const unsigned* u, const unsigned* v, const double* w, double gamma,
@@ -550,8 +584,8 @@ int iqsh_qaoa_layer(iqsh_handle h, unsigned ne, const unsigned* u, const unsigne
   CutSpec cs;
   if (int rc = iqsi::build_cut_spec(h, ne, u, v, w, gamma, &cs)) return rc;
   std::lock_guard<std::mutex> lk(g_lm);
-  return h->dtype == IQSH_C128 ? do_layer<double, false>(h, nullptr, mixer, nullptr, &cs, nullptr, nullptr)
-                               : do_layer<float, false>(h, nullptr, mixer, nullptr, &cs, nullptr, nullptr);
+  return h->dtype == IQSH_C128 ? do_layer<double, false, false>(h, nullptr, mixer, nullptr, &cs, nullptr, nullptr)
+                               : do_layer<float, false, false>(h, nullptr, mixer, nullptr, &cs, nullptr, nullptr);
 }
 
 int iqsh_qaoa_layer_adjoint(iqsh_handle psi, iqsh_handle lam, unsigned ne, const unsigned* u, const unsigned* v,
@@ -567,8 +601,7 @@ int iqsh_qaoa_layer_adjoint(iqsh_handle psi, iqsh_handle lam, unsigned ne, const
   if (int rc = iqsi::build_cut_spec(psi, ne, u, v, w, -gamma, &cs)) return rc;
   std::lock_guard<std::mutex> lk(g_lm);
   if (int rc = part_buf()) return rc;
-  return psi->dtype == IQSH_C128 ? do_layer<double, true>(psi, lam, mixer_inv, G, &cs, dmixer, dgamma)
-                                 : do_layer<float, true>(psi, lam, mixer_inv, G, &cs, dmixer, dgamma);
+  return adjoint_layer(psi, lam, mixer_inv, G, &cs, dmixer, dgamma);
 }
 
 int iqsh_adjoint_1q_all(iqsh_handle psi, iqsh_handle lam, const double Uinv[8], const double* G, double* grad)
@@ -579,8 +612,7 @@ int iqsh_adjoint_1q_all(iqsh_handle psi, iqsh_handle lam, const double Uinv[8], 
   if (psi->n < kMinLayerQubits) return iqsi::fail(IQSH_ERR_NOTSUP, "layer kernels need >= %u qubits", kMinLayerQubits);
   std::lock_guard<std::mutex> lk(g_lm);
   if (int rc = part_buf()) return rc;
-  return psi->dtype == IQSH_C128 ? do_layer<double, true>(psi, lam, Uinv, G, nullptr, grad, nullptr)
-                                 : do_layer<float, true>(psi, lam, Uinv, G, nullptr, grad, nullptr);
+  return adjoint_layer(psi, lam, Uinv, G, nullptr, grad, nullptr);
 }
 
 }  // extern "C"

@@ -47,14 +47,19 @@ def test_layer_forward(dt, n):
     assert np.abs(sv.download() - h).max() < TOL[dt]
 
 
+@pytest.mark.parametrize("gen", ["general", "X"])
 @pytest.mark.parametrize("dt", ["c128", "c64"])
 @pytest.mark.parametrize("n", [10, 12, 15, 20])
-def test_layer_adjoint(dt, n):
+def test_layer_adjoint(dt, n, gen):
+    """Any generator, and G = X exactly (the kernels' two-product form of the term)."""
     psi = _state(n, dt, 2 * n)
     lam = _state(n, dt, 2 * n + 1)
     u = _unitary(7 * n)
     uinv = u.conj().T
-    G = np.array([[0.3, 1.2 - 0.4j], [1.2 + 0.4j, -0.7]])  # any 2x2, Hermitian here
+    if gen == "X":
+        G = np.array([[0, 1], [1, 0]], dtype=complex)
+    else:
+        G = np.array([[0.3, 1.2 - 0.4j], [1.2 + 0.4j, -0.7]])  # any 2x2, Hermitian here
     # reference value: sum_q 2 Im <lam| G_q |psi> on the host (FP64)
     want = 0.0
     for q in range(n):

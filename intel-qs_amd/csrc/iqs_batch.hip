@@ -6,11 +6,11 @@
 // Adjoint method: psi = final state, lam = O psi with O = sum_q Z_q (diagonal, o(x) =
 // n - 2 popcount(x)); walking the gates backwards, the rotation exp(-i a P/2) contributes
 // dC/da = Im <lam|P|psi> at the point just after it, then it is undone on both vectors.
-// With psi', lam' taken after RZ on the same qubit (other qubits' rotations commute):
-//   dC/dphi   = Im <lam'|Z_q|psi'>
-//   dC/dtheta = cos(phi) Im <lam'|Y_q|psi'> - sin(phi) Im <lam'|X_q|psi'>
-// since RZ(phi) Y RZ(phi)^dagger = cos(phi) Y - sin(phi) X.  The three sums come from the
-// same amplitude pairs, so each qubit costs one pass over its pairs.
+// The kernel applies a layer as U_l = CZ_l . prod_q RZ_q(phi_lq) . prod_q RY_q(theta_lq):
+// real 2x2 rotations per pair for the RYs, and per register phase one diagonal per
+// amplitude holding RZ factors and CZ signs (see Tile).  So
+//   dC/dtheta = Im <lam|Y_q|psi>  just after the RY (one product per pair),
+//   dC/dphi   = Im <lam|Z_q|psi>  at the diagonal that holds the RZ (one per amplitude).
 //
 // Work decomposition (M = tile qubits, R = amplitudes per thread, NT = 2^M / R threads):
 //  * n <= M: the whole instance is one task.  Its state lives in registers (padded to 2^M
@@ -23,12 +23,14 @@
 //    so one pass over the state applies about one layer: layer l's rotations are split
 //    between passes l and l+1 (they commute), and CZ of layer l sits between them.
 //    Forward passes j = 0..d-1, a turnaround pass j = d (forward, cost, lam, backward),
-//    backward passes j = d-1..0 on psi and lam: 2d+1 steps of 2^(n-M) tile tasks.
+//    backward passes j = d-1..0 on psi and lam: 2d+1 steps of 2^(n-M) tile tasks.  The RZs
+//    of the qubits a pass rotates last (its R_j) move to the next pass's first diagonal: an
+//    RZ commutes with everything between its qubit's RY of the same layer and the next.
 //  * Inside a task the registers cycle through phases: phase k holds a run of log2(R)
 //    consecutive local bits (runs overlap by one bit), gates on those bits are register-
 //    local, and the layout changes through one LDS transpose per phase.  A CZ pair (i,i+1)
-//    is applied in the phase where its upper gated qubit first appears, after the R_{j-1}
-//    of both and before their R_j (a CZ is diagonal: it only needs the global index).
+//    is applied in the phase where its upper gated qubit first appears, after the RY_{j-1}
+//    of both and before their RY_j (a CZ is diagonal: it only needs the global index).
 //  * Scheduling: a global queue of tasks in a host-chosen order; a workgroup takes the
 //    next task with one atomic, and a tile task of step s waits until all tiles of step
 //    s-1 of its instance have arrived.  Every task waits only on tasks earlier in the
@@ -86,8 +88,7 @@ template <typename T> struct HeaArgs {
   uint32_t flags;
   uint32_t* ctl;
   V<T>* arena;
-  const V<T>* mats;  // 4 entries per pair: U = RZ(phi) RY(theta), row-major
-  const double* ang;
+  const V<T>* mats;  // 4 entries per pair: (cos, sin) of theta/2 and of phi/2, then 2 unused
   double* grad;
   double* cost;
   double* part;
@@ -1064,7 +1065,6 @@ int enqueue_t(iqsh_hea* h, int mode, hipStream_t st)
   a.ctl = h->ctl;
   a.arena = (V<T>*)h->arena;
   a.mats = (const V<T>*)h->mats;
-  a.ang = h->ang;
   a.grad = h->grad;
   a.cost = h->cost;
   a.part = h->part;

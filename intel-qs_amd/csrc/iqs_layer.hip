@@ -95,6 +95,8 @@ __device__ __forceinline__ double gen_term(const V& y, const Mat2<double>& g, co
 {
   if (GX) {
     const V o = hi ? x0 : x1;
+    // complex64: the product in FP32 (of FP32 amplitudes), summed in FP64 by the caller
+    if constexpr (sizeof(o.x) == 4) return (double)(y.x * o.y - y.y * o.x);
     return (double)y.x * (double)o.y - (double)y.y * (double)o.x;
   }
   return hi ? im_term(y, g.m10, x0, g.m11, x1) : im_term(y, g.m00, x0, g.m01, x1);

@@ -809,6 +809,13 @@ def main() -> int:
     ref0.close()
     log(f"brick circuit undone: max |psi - |0>| = {roundtrip:.3e}")
 
+    # ---- attainable streaming rate on this GPU: the library's in-place scale of the whole
+    # state (k_scale, iqsh_scale: one read and one write per amplitude, no partner), the
+    # simplest read-modify-write pass; the dense 1q update is compared against it ----
+    stream_ms = _time_ms(lambda: sv.scale(-1.0), 5, stream, torch)
+    attainable = 2 * S / stream_ms / 1e6
+    log(f"in-place scale of the state: {attainable:.0f} GB/s")
+
     # ---- per-kind, permutation and reduction sweeps over every position ----
     sweep = sweep_c64 = perm = red = None
     if not args.no_sweep:
@@ -860,7 +867,10 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profile(n),
                      "kernel": "k_pair<double,1,false,true,128,2,4,SL> (dense 1q update, target >= 6: H, RX, RY; "
                                "sub-ranges 2^SL items apart, SL=10 for targets 6-10, 11 above)",
-                     "avg_launch_ms": 1e3 * dense_avg, "launches": len(dense)},
+                     "avg_launch_ms": 1e3 * dense_avg, "launches": len(dense),
+                     "attainable": attainable, "frac_of_attainable": achieved / attainable,
+                     "attainable_kernel": "k_scale (iqsh_scale on the same state: in-place read and write of "
+                                          "every amplitude, measured in this run)"},
         "per_kind_avg_ms": {k: 1e3 * sum(v) / len(v) for k, v in per_kind.items()},
         "hbm_frac_1q": achieved / HBM_PEAK_GBS,
         "hbm_frac_1q_all_targets": (2 * S / (sum(dense + per_kind.get("dense1q_lane", [])) /

@@ -102,6 +102,28 @@ __device__ __forceinline__ double gen_term(const V& y, const Mat2<double>& g, co
   return hi ? im_term(y, g.m10, x0, g.m11, x1) : im_term(y, g.m00, x0, g.m01, x1);
 }
 
+// Kernel form flags: kFormGX, the generator is X (gen_term's two-product form); kFormRX,
+// the 2x2 has a real diagonal and an imaginary off-diagonal (the mixer RX(2 beta) and its
+// inverse), so a row of the product is d x_same + i o x_other: four products instead of eight.
+constexpr int kFormGX = 1, kFormRX = 2;
+
+// (The lane passes keep the general product written out at the call site: behind this helper
+// the compiler gave their forward kernels 86 instead of 56 VGPRs.)
+template <int F, typename T, typename V>
+__device__ __forceinline__ V urow(const Mat2<T>& u, const V& x0, const V& x1, bool hi)
+{
+  if constexpr ((F & kFormRX) != 0) {
+    const T d = hi ? u.m11.r : u.m00.r, o = hi ? u.m10.i : u.m01.i;
+    const V xs = hi ? x1 : x0, xo = hi ? x0 : x1;
+    V r;
+    r.x = d * xs.x - o * xo.y;
+    r.y = d * xs.y + o * xo.x;
+    return r;
+  } else {
+    return hi ? mat_row(u.m10, x0, u.m11, x1) : mat_row(u.m00, x0, u.m01, x1);
+  }
+}
+
 // Workgroup sum of v (fixed order: lanes by xor-shuffle, then the waves in order) stored
 // at out[slot] by thread 0.
 __device__ __forceinline__ void wg_sum_store(double v, double* out, uint64_t slot)
@@ -151,7 +173,7 @@ __device__ __forceinline__ unsigned row_index(unsigned tid, int j)
 
 // The gate on the register positions of a lane pass (bit b of j: position 6 + b), with the
 // generator terms of the adjoint taken before each position is undone.
-template <typename T, bool ADJ, int P, bool GX, typename V>
+template <typename T, bool ADJ, int P, int F, typename V>
 __device__ __forceinline__ void reg_positions(V* x, V* y, const Mat2<T>& u, const Mat2<double>& g, double& acc)
 {
 #pragma unroll
@@ -161,20 +183,20 @@ __device__ __forceinline__ void reg_positions(V* x, V* y, const Mat2<T>& u, cons
       if ((j >> b) & 1) continue;
       const int j1 = j | (1 << b);
       const V a = x[j], c = x[j1];
-      if (ADJ) acc += gen_term<GX>(y[j], g, a, c, false) + gen_term<GX>(y[j1], g, a, c, true);
-      x[j] = mat_row(u.m00, a, u.m01, c);
-      x[j1] = mat_row(u.m10, a, u.m11, c);
+      if (ADJ) acc += gen_term<(F & kFormGX) != 0>(y[j], g, a, c, false) + gen_term<(F & kFormGX) != 0>(y[j1], g, a, c, true);
+      x[j] = urow<F>(u, a, c, false);
+      x[j1] = urow<F>(u, a, c, true);
       if (ADJ) {
         const V ya = y[j], yc = y[j1];
-        y[j] = mat_row(u.m00, ya, u.m01, yc);
-        y[j1] = mat_row(u.m10, ya, u.m11, yc);
+        y[j] = urow<F>(u, ya, yc, false);
+        y[j1] = urow<F>(u, ya, yc, true);
       }
     }
 }
 
 // Lane pass over positions 0 .. kLowQ + log2(kLowP) - 1 (shuffles for 0-5, registers above).
 // N a multiple of kLB * kLowP.
-template <typename T, bool ADJ, bool GX>
+template <typename T, bool ADJ, int F>
 __global__ __launch_bounds__(kLB) void k_layer_low(typename Vec<T>::t* __restrict__ psi, typename Vec<T>::t* __restrict__ lam,
                                                    uint64_t N, Mat2<T> u, Mat2<double> g, double* __restrict__ part)
 {
@@ -204,15 +226,15 @@ __global__ __launch_bounds__(kLB) void k_layer_low(typename Vec<T>::t* __restric
         const V xp = shfl_v(x[j], msk);
         const V x0 = hi ? xp : x[j], x1 = hi ? x[j] : xp;
         if (ADJ) {
-          acc += gen_term<GX>(y[j], g, x0, x1, hi);
+          acc += gen_term<(F & kFormGX) != 0>(y[j], g, x0, x1, hi);
           const V yp = shfl_v(y[j], msk);
           const V y0 = hi ? yp : y[j], y1 = hi ? y[j] : yp;
-          y[j] = hi ? mat_row(u.m10, y0, u.m11, y1) : mat_row(u.m00, y0, u.m01, y1);
+          y[j] = (F & kFormRX) ? urow<F>(u, y0, y1, hi) : (hi ? mat_row(u.m10, y0, u.m11, y1) : mat_row(u.m00, y0, u.m01, y1));
         }
-        x[j] = hi ? mat_row(u.m10, x0, u.m11, x1) : mat_row(u.m00, x0, u.m01, x1);
+        x[j] = (F & kFormRX) ? urow<F>(u, x0, x1, hi) : (hi ? mat_row(u.m10, x0, u.m11, x1) : mat_row(u.m00, x0, u.m01, x1));
       }
     }
-    reg_positions<T, ADJ, kLowP, GX>(x, y, u, g, acc);
+    reg_positions<T, ADJ, kLowP, F>(x, y, u, g, acc);
 #pragma unroll
     for (int j = 0; j < kLowP; ++j) {
       stg(x[j], &psi[r0 + row_index<kLowP>(threadIdx.x, j)]);
@@ -234,7 +256,7 @@ __global__ __launch_bounds__(kLB) void k_layer_low(typename Vec<T>::t* __restric
 // takes E(lo) of its P in-row offsets once, and E(row) comes from the row's set bits, the
 // same for the whole workgroup -- no per-workgroup tables, so short workgroups as in the
 // plain lane pass.  W: weighted cuts, summed from byte tables of weight sums in LDS.
-template <typename T, bool ADJ, bool W, bool GX>
+template <typename T, bool ADJ, bool W, int F>
 __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict__ psi, typename Vec<T>::t* __restrict__ lam,
                                                   uint64_t N, Mat2<T> u, Mat2<double> g, CutSpec cs,
                                                   double* __restrict__ part)
@@ -309,15 +331,15 @@ __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict
         const V xp = shfl_v(x[j], msk);
         const V x0 = hi ? xp : x[j], x1 = hi ? x[j] : xp;
         if (ADJ) {
-          acc += gen_term<GX>(y[j], g, x0, x1, hi);
+          acc += gen_term<(F & kFormGX) != 0>(y[j], g, x0, x1, hi);
           const V yp = shfl_v(y[j], msk);
           const V y0 = hi ? yp : y[j], y1 = hi ? y[j] : yp;
-          y[j] = hi ? mat_row(u.m10, y0, u.m11, y1) : mat_row(u.m00, y0, u.m01, y1);
+          y[j] = (F & kFormRX) ? urow<F>(u, y0, y1, hi) : (hi ? mat_row(u.m10, y0, u.m11, y1) : mat_row(u.m00, y0, u.m01, y1));
         }
-        x[j] = hi ? mat_row(u.m10, x0, u.m11, x1) : mat_row(u.m00, x0, u.m01, x1);
+        x[j] = (F & kFormRX) ? urow<F>(u, x0, x1, hi) : (hi ? mat_row(u.m10, x0, u.m11, x1) : mat_row(u.m00, x0, u.m01, x1));
       }
     }
-    reg_positions<T, ADJ, P, GX>(x, y, u, g, acc);
+    reg_positions<T, ADJ, P, F>(x, y, u, g, acc);
 #pragma unroll
     for (int j = 0; j < P; ++j) {
       if (ADJ) {
@@ -343,7 +365,7 @@ __global__ __launch_bounds__(kLB) void k_qaoa_low(typename Vec<T>::t* __restrict
 // TILED: one group per thread, grid = items / kLB workgroups (items a power of two >=
 // kRegTileMin), one partial per workgroup at part[block_id()]; otherwise a grid-stride loop
 // over one contiguous chunk of groups per workgroup.
-template <typename T, bool ADJ, int M, bool TILED, bool GX>
+template <typename T, bool ADJ, int M, bool TILED, int F>
 __global__ __launch_bounds__(kLB) void k_layer_reg(typename Vec<T>::t* __restrict__ psi, typename Vec<T>::t* __restrict__ lam,
                                                    uint64_t items, Ins<M> ins, Mat2<T> u, Mat2<double> g,
                                                    double* __restrict__ part)
@@ -390,7 +412,8 @@ __global__ __launch_bounds__(kLB) void k_layer_reg(typename Vec<T>::t* __restric
         for (int j = 0; j < D; ++j) {
           if ((j >> b) & 1) continue;
           const int j1 = j | (1 << b);
-          acc += gen_term<GX>(y[j], g, x[j], x[j1], false) + gen_term<GX>(y[j1], g, x[j], x[j1], true);
+          acc += gen_term<(F & kFormGX) != 0>(y[j], g, x[j], x[j1], false) +
+                 gen_term<(F & kFormGX) != 0>(y[j1], g, x[j], x[j1], true);
         }
     }
 #pragma unroll
@@ -400,12 +423,12 @@ __global__ __launch_bounds__(kLB) void k_layer_reg(typename Vec<T>::t* __restric
         if ((j >> b) & 1) continue;
         const int j1 = j | (1 << b);
         const V a = x[j], c = x[j1];
-        x[j] = mat_row(u.m00, a, u.m01, c);
-        x[j1] = mat_row(u.m10, a, u.m11, c);
+        x[j] = urow<F>(u, a, c, false);
+        x[j1] = urow<F>(u, a, c, true);
         if (ADJ) {
           const V ya = y[j], yc = y[j1];
-          y[j] = mat_row(u.m00, ya, u.m01, yc);
-          y[j1] = mat_row(u.m10, ya, u.m11, yc);
+          y[j] = urow<F>(u, ya, yc, false);
+          y[j1] = urow<F>(u, ya, yc, true);
         }
       }
 #pragma unroll
@@ -450,7 +473,7 @@ int finish_pass(unsigned grid, double* acc)
   return IQSH_OK;
 }
 
-template <typename T, bool ADJ, int M, bool GX>
+template <typename T, bool ADJ, int M, int F>
 int reg_pass(iqsh_handle h, iqsh_handle lam, unsigned p0, const Mat2<T>& u, const Mat2<double>& g, double* acc)
 {
   using V = typename Vec<T>::t;
@@ -466,14 +489,14 @@ int reg_pass(iqsh_handle h, iqsh_handle lam, unsigned p0, const Mat2<T>& u, cons
     if (ADJ)
       if (int rc = part_buf(G + 1)) return rc;
     const dim3 grid((unsigned)std::min<uint64_t>(G, 65536), (unsigned)((G + 65535) / 65536), 1);
-    k_layer_reg<T, ADJ, M, true, GX><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, items, ins, u, g, g_lw.part);
+    k_layer_reg<T, ADJ, M, true, F><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, items, ins, u, g, g_lw.part);
     IQSI_TRY(hipGetLastError());
     if (!ADJ) return IQSH_OK;
     double* out[1] = {acc};
     return device_sums(G, 1, out);
   }
   const unsigned grid = grid_for_work(items);
-  k_layer_reg<T, ADJ, M, false, GX><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, items, ins, u, g, g_lw.part);
+  k_layer_reg<T, ADJ, M, false, F><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, items, ins, u, g, g_lw.part);
   if (!ADJ) {
     IQSI_TRY(hipGetLastError());
     return IQSH_OK;
@@ -481,7 +504,7 @@ int reg_pass(iqsh_handle h, iqsh_handle lam, unsigned p0, const Mat2<T>& u, cons
   return finish_pass(grid, acc);
 }
 
-template <typename T, bool ADJ, bool GX>
+template <typename T, bool ADJ, int F>
 int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm, const CutSpec* cs, double* grad,
              double* grad_phase)
 {
@@ -507,11 +530,11 @@ int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm,
       if (int rc = part_buf(2 * (uint64_t)grid + 2)) return rc;
     V* lp = ADJ ? (V*)lam->d : nullptr;
     if (cs && cs->weighted)
-      k_qaoa_low<T, ADJ, true, GX><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, h->size(), u, g, *cs, g_lw.part);
+      k_qaoa_low<T, ADJ, true, F><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, h->size(), u, g, *cs, g_lw.part);
     else if (cs)
-      k_qaoa_low<T, ADJ, false, GX><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, h->size(), u, g, *cs, g_lw.part);
+      k_qaoa_low<T, ADJ, false, F><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, lp, h->size(), u, g, *cs, g_lw.part);
     else
-      k_layer_low<T, ADJ, GX><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, ADJ ? (V*)lam->d : nullptr, h->size(), u, g,
+      k_layer_low<T, ADJ, F><<<grid, kLB, 0, iqsi::stream()>>>((V*)h->d, ADJ ? (V*)lam->d : nullptr, h->size(), u, g,
                                                              g_lw.part);
     IQSI_TRY(hipGetLastError());
     if (!ADJ) return IQSH_OK;
@@ -530,11 +553,11 @@ int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm,
   for (unsigned i = 0, p = lowq; i < npass; ++i) {
     const unsigned m = rest / npass + (i < rest % npass ? 1 : 0);
     int rc = IQSH_OK;
-    if (m == 5) rc = reg_pass<T, false, 5, false>(h, lam, p, u, g, &acc);
-    else if (m == 4) rc = reg_pass<T, ADJ, 4, GX>(h, lam, p, u, g, &acc);
-    else if (m == 3) rc = reg_pass<T, ADJ, 3, GX>(h, lam, p, u, g, &acc);
-    else if (m == 2) rc = reg_pass<T, ADJ, 2, GX>(h, lam, p, u, g, &acc);
-    else rc = reg_pass<T, ADJ, 1, GX>(h, lam, p, u, g, &acc);
+    if (m == 5) rc = reg_pass<T, false, 5, 0>(h, lam, p, u, g, &acc);
+    else if (m == 4) rc = reg_pass<T, ADJ, 4, F>(h, lam, p, u, g, &acc);
+    else if (m == 3) rc = reg_pass<T, ADJ, 3, F>(h, lam, p, u, g, &acc);
+    else if (m == 2) rc = reg_pass<T, ADJ, 2, F>(h, lam, p, u, g, &acc);
+    else rc = reg_pass<T, ADJ, 1, F>(h, lam, p, u, g, &acc);
     if (rc) return rc;
     p += m;
   }
@@ -545,17 +568,26 @@ int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm,
   return IQSH_OK;
 }
 
-// Backward step through a layer: the kernels with G = X when the generator is exactly X.
-int adjoint_layer(iqsh_handle psi, iqsh_handle lam, const double* uinv, const double* G, const CutSpec* cs,
-                  double* grad, double* grad_phase)
+// Backward step through a layer: the kernels with G = X when the generator is exactly X,
+// and with the RX-form product as well when uinv has a real diagonal and an imaginary
+// off-diagonal (the QAOA mixer's inverse).
+template <typename T>
+int adjoint_layer_t(iqsh_handle psi, iqsh_handle lam, const double* uinv, const double* G, const CutSpec* cs,
+                    double* grad, double* grad_phase)
 {
   static const double X[8] = {0, 0, 1, 0, 1, 0, 0, 0};
   const bool gx = std::equal(G, G + 8, X);
-  if (psi->dtype == IQSH_C128)
-    return gx ? do_layer<double, true, true>(psi, lam, uinv, G, cs, grad, grad_phase)
-              : do_layer<double, true, false>(psi, lam, uinv, G, cs, grad, grad_phase);
-  return gx ? do_layer<float, true, true>(psi, lam, uinv, G, cs, grad, grad_phase)
-            : do_layer<float, true, false>(psi, lam, uinv, G, cs, grad, grad_phase);
+  const bool rx = uinv[1] == 0 && uinv[2] == 0 && uinv[4] == 0 && uinv[7] == 0;
+  if (gx && rx) return do_layer<T, true, kFormGX | kFormRX>(psi, lam, uinv, G, cs, grad, grad_phase);
+  if (gx) return do_layer<T, true, kFormGX>(psi, lam, uinv, G, cs, grad, grad_phase);
+  return do_layer<T, true, 0>(psi, lam, uinv, G, cs, grad, grad_phase);
+}
+
+int adjoint_layer(iqsh_handle psi, iqsh_handle lam, const double* uinv, const double* G, const CutSpec* cs,
+                  double* grad, double* grad_phase)
+{
+  return psi->dtype == IQSH_C128 ? adjoint_layer_t<double>(psi, lam, uinv, G, cs, grad, grad_phase)
+                                 : adjoint_layer_t<float>(psi, lam, uinv, G, cs, grad, grad_phase);
 }
 
 // The lane pass needs whole rows of kLB * kLowP amplitudes with lanes spanning >= 64.
@@ -573,8 +605,8 @@ int iqsh_apply_1q_all(iqsh_handle h, const double m[8])
   if (!m) return iqsi::fail(IQSH_ERR_ARG, "null matrix");
   if (h->n < kMinLayerQubits) return iqsi::fail(IQSH_ERR_NOTSUP, "layer kernels need >= %u qubits", kMinLayerQubits);
   std::lock_guard<std::mutex> lk(g_lm);
-  return h->dtype == IQSH_C128 ? do_layer<double, false, false>(h, nullptr, m, nullptr, nullptr, nullptr, nullptr)
-                               : do_layer<float, false, false>(h, nullptr, m, nullptr, nullptr, nullptr, nullptr);
+  return h->dtype == IQSH_C128 ? do_layer<double, false, 0>(h, nullptr, m, nullptr, nullptr, nullptr, nullptr)
+                               : do_layer<float, false, 0>(h, nullptr, m, nullptr, nullptr, nullptr, nullptr);
 }
 
 int iqsh_qaoa_layer(iqsh_handle h, unsigned ne, const unsigned* u, const unsigned* v, const double* w, double gamma,
@@ -586,8 +618,8 @@ int iqsh_qaoa_layer(iqsh_handle h, unsigned ne, const unsigned* u, const unsigne
   CutSpec cs;
   if (int rc = iqsi::build_cut_spec(h, ne, u, v, w, gamma, &cs)) return rc;
   std::lock_guard<std::mutex> lk(g_lm);
-  return h->dtype == IQSH_C128 ? do_layer<double, false, false>(h, nullptr, mixer, nullptr, &cs, nullptr, nullptr)
-                               : do_layer<float, false, false>(h, nullptr, mixer, nullptr, &cs, nullptr, nullptr);
+  return h->dtype == IQSH_C128 ? do_layer<double, false, 0>(h, nullptr, mixer, nullptr, &cs, nullptr, nullptr)
+                               : do_layer<float, false, 0>(h, nullptr, mixer, nullptr, &cs, nullptr, nullptr);
 }
 
 int iqsh_qaoa_layer_adjoint(iqsh_handle psi, iqsh_handle lam, unsigned ne, const unsigned* u, const unsigned* v,

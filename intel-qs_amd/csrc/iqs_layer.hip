@@ -553,7 +553,7 @@ int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm,
   for (unsigned i = 0, p = lowq; i < npass; ++i) {
     const unsigned m = rest / npass + (i < rest % npass ? 1 : 0);
     int rc = IQSH_OK;
-    if (m == 5) rc = reg_pass<T, false, 5, 0>(h, lam, p, u, g, &acc);
+    if (m == 5) rc = reg_pass<T, false, 5, ADJ ? 0 : F>(h, lam, p, u, g, &acc);
     else if (m == 4) rc = reg_pass<T, ADJ, 4, F>(h, lam, p, u, g, &acc);
     else if (m == 3) rc = reg_pass<T, ADJ, 3, F>(h, lam, p, u, g, &acc);
     else if (m == 2) rc = reg_pass<T, ADJ, 2, F>(h, lam, p, u, g, &acc);
@@ -583,6 +583,15 @@ int adjoint_layer_t(iqsh_handle psi, iqsh_handle lam, const double* uinv, const 
   return do_layer<T, true, 0>(psi, lam, uinv, G, cs, grad, grad_phase);
 }
 
+// Forward layer: the RX-form kernels when m has a real diagonal and an imaginary off-diagonal.
+template <typename T>
+int forward_layer_t(iqsh_handle h, const double* m, const CutSpec* cs)
+{
+  const bool rx = m[1] == 0 && m[2] == 0 && m[4] == 0 && m[7] == 0;
+  return rx ? do_layer<T, false, kFormRX>(h, nullptr, m, nullptr, cs, nullptr, nullptr)
+            : do_layer<T, false, 0>(h, nullptr, m, nullptr, cs, nullptr, nullptr);
+}
+
 int adjoint_layer(iqsh_handle psi, iqsh_handle lam, const double* uinv, const double* G, const CutSpec* cs,
                   double* grad, double* grad_phase)
 {
@@ -605,8 +614,7 @@ int iqsh_apply_1q_all(iqsh_handle h, const double m[8])
   if (!m) return iqsi::fail(IQSH_ERR_ARG, "null matrix");
   if (h->n < kMinLayerQubits) return iqsi::fail(IQSH_ERR_NOTSUP, "layer kernels need >= %u qubits", kMinLayerQubits);
   std::lock_guard<std::mutex> lk(g_lm);
-  return h->dtype == IQSH_C128 ? do_layer<double, false, 0>(h, nullptr, m, nullptr, nullptr, nullptr, nullptr)
-                               : do_layer<float, false, 0>(h, nullptr, m, nullptr, nullptr, nullptr, nullptr);
+  return h->dtype == IQSH_C128 ? forward_layer_t<double>(h, m, nullptr) : forward_layer_t<float>(h, m, nullptr);
 }
 
 int iqsh_qaoa_layer(iqsh_handle h, unsigned ne, const unsigned* u, const unsigned* v, const double* w, double gamma,
@@ -618,8 +626,7 @@ int iqsh_qaoa_layer(iqsh_handle h, unsigned ne, const unsigned* u, const unsigne
   CutSpec cs;
   if (int rc = iqsi::build_cut_spec(h, ne, u, v, w, gamma, &cs)) return rc;
   std::lock_guard<std::mutex> lk(g_lm);
-  return h->dtype == IQSH_C128 ? do_layer<double, false, 0>(h, nullptr, mixer, nullptr, &cs, nullptr, nullptr)
-                               : do_layer<float, false, 0>(h, nullptr, mixer, nullptr, &cs, nullptr, nullptr);
+  return h->dtype == IQSH_C128 ? forward_layer_t<double>(h, mixer, &cs) : forward_layer_t<float>(h, mixer, &cs);
 }
 
 int iqsh_qaoa_layer_adjoint(iqsh_handle psi, iqsh_handle lam, unsigned ne, const unsigned* u, const unsigned* v,

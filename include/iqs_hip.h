@@ -215,13 +215,15 @@ int iqsh_adjoint_1q(iqsh_handle psi, iqsh_handle lam, unsigned pos, const double
 
 /* ---- layers of one gate on every qubit -------------------------------------------------
  * psi := m (x) m (x) ... (x) m psi (the same 2x2 matrix on every position) in
- * 1 + ceil((n - 6) / 4) passes instead of n.  Replaces the per-qubit loop that applies a
+ * 1 + ceil((n - 8) / 5) passes instead of n (a matrix with a real diagonal and an imaginary
+ * off-diagonal, such as RX, takes shorter kernels).  Replaces the per-qubit loop that applies a
  * QAOA mixer, RX(2 beta) on every qubit, after ImplementQaoaLayerBasedOnCostFunction
  * (src/qaoa_features.cpp:73-110 users; src/qureg_apply1qubitgate.cpp:277-285).  n >= 10. */
 int iqsh_apply_1q_all(iqsh_handle h, const double m[8]);
 /* Backward step through such a layer for both states: *grad := sum_q 2 Im <lam| G_q |psi>
  * (G on position q, each term as iqsh_adjoint_1q would take it), then psi := Uinv^(x)n psi,
- * lam := Uinv^(x)n lam.  n >= 10. */
+ * lam := Uinv^(x)n lam, in 1 + ceil((n - 8) / 4) passes (G equal to X exactly, and Uinv of
+ * the RX form, take shorter kernels).  n >= 10. */
 int iqsh_adjoint_1q_all(iqsh_handle psi, iqsh_handle lam, const double Uinv[8], const double *G, double *grad);
 
 /* One QAOA layer, psi := mixer^(x)n exp(-i gamma C) psi with C the cut of the edge list (as

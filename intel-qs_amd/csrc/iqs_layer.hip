@@ -31,7 +31,7 @@ using namespace iqsk;
 namespace {
 
 constexpr int kLB = 256;          // threads per workgroup
-constexpr int kLowQ = 6;          // positions handled by the lane-shuffle pass
+constexpr int kLowQ = 6;          // positions exchanged by lane shuffles in the lane pass (lane bits)
 constexpr int kLowP = 4;          // amplitudes per lane in the lane-shuffle pass
 constexpr int kQaoaAdjP = 4;      // amplitudes of psi and of lambda per lane in the QAOA adjoint lane pass
 constexpr int kRegQ = 4;          // positions per register pass (adjoint: psi and lambda)

@@ -481,9 +481,10 @@ int reg_pass(iqsh_handle h, iqsh_handle lam, unsigned p0, const Mat2<T>& u, cons
   for (int b = 0; b < M; ++b) ins.p[b] = p0 + b;
   const uint64_t items = h->size() >> M;
   V* lp = ADJ ? (V*)lam->d : nullptr;
-  // The complex128 adjoint pass over 4 positions holds 32 amplitudes per thread and keeps the
-  // grid-stride form (28 qubits: 3.21 ms against 3.29 ms tiled).
-  const bool tiled = items >= kRegTileMin && !(ADJ && M == 4 && sizeof(T) == 8);
+  // The complex128 adjoint pass over 4 positions holds 32 amplitudes per thread; in its
+  // general form it keeps the grid-stride loop (28 qubits: 3.21 ms against 3.29 ms tiled),
+  // in the RX/X form the tiled one (3.05 ms against 3.23).
+  const bool tiled = items >= kRegTileMin && !(ADJ && M == 4 && sizeof(T) == 8 && F != (kFormGX | kFormRX));
   if (tiled) {
     const uint64_t G = items / kLB;
     if (ADJ)

@@ -37,10 +37,11 @@ constexpr int kQaoaAdjP = 4;      // amplitudes of psi and of lambda per lane in
 constexpr int kRegQ = 4;          // positions per register pass (adjoint: psi and lambda)
 constexpr int kRegQF = 5;         // positions per register pass (forward)
 constexpr unsigned kMaxGrid = 2048;    // workgroups of the grid-stride register passes (small states)
-// Lane passes: kLowRows rows of kLB * kLowP amplitudes per workgroup (28-qubit complex128:
-// forward 4.84 -> 5.45 TB/s, adjoint 4.07 -> 4.35 against 2048 workgroups with 128 rows each;
-// the QAOA form also builds its cut tables per workgroup, a few entries per thread).
-constexpr uint64_t kLowRows = 4;
+// Lane passes: kLowRowsFwd, kLowRowsAdj rows of kLB * P amplitudes per workgroup
+// (28-qubit complex128 with the phase, RX/X forms: forward 1.43 ms with one row, 1.48 with
+// two, 1.56 with four; adjoint 3.01 ms with two rows, 3.09 with one, 3.21 with four; 2048
+// workgroups of 128 rows each streamed 10-20% slower).
+constexpr uint64_t kLowRowsFwd = 1, kLowRowsAdj = 2;
 constexpr uint64_t kLowMaxGrid = uint64_t(1) << 22;
 // Register passes over >= kRegTileMin groups run one group per thread and one tile of kLB
 // groups per workgroup (XCD-contiguous, split into kRegSub runs of kLB / kRegSub groups
@@ -522,11 +523,11 @@ int do_layer(iqsh_handle h, iqsh_handle lam, const double* um, const double* gm,
   // Forward: the lane pass first (it carries the phase layer, applied before the mixer);
   // adjoint: last (the phase is undone after the whole mixer).
   auto low_pass = [&]() -> int {
-    // kLowRows rows per workgroup.  The weighted QAOA form builds byte tables of weight sums
+    // kLowRowsFwd or kLowRowsAdj rows per workgroup.  The weighted QAOA form builds byte tables of weight sums
     // per workgroup: a persistent grid of <= kMaxGrid workgroups for it.
     const uint64_t rows = h->size() / (kLB * (uint64_t)lowp);
     const unsigned grid = (cs && cs->weighted) ? (unsigned)std::min<uint64_t>(rows, kMaxGrid)
-                                               : (unsigned)std::min<uint64_t>(std::max<uint64_t>(1, rows / kLowRows), kLowMaxGrid);
+                                               : (unsigned)std::min<uint64_t>(std::max<uint64_t>(1, rows / (ADJ ? kLowRowsAdj : kLowRowsFwd)), kLowMaxGrid);
     if (ADJ)
       if (int rc = part_buf(2 * (uint64_t)grid + 2)) return rc;
     V* lp = ADJ ? (V*)lam->d : nullptr;

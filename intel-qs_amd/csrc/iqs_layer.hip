@@ -6,10 +6,11 @@
 //   * one lane pass for positions 0..7: each lane holds 4 amplitudes 64 apart, partners at
 //     lane ^ 2^q (positions 0-5) come through __shfl_xor, positions 6 and 7 pair the
 //     lane's own registers, and every load/store is a contiguous run of 64 amplitudes;
-//   * passes of up to four positions each above that: a thread holds the 2^m amplitudes of
-//     one group in registers (consecutive threads take consecutive groups, so every
-//     instruction is a contiguous run) and applies the m rotations there.
-// At 28 qubits: 6 passes instead of 28.
+//   * passes of up to five positions each above that (four in the adjoint, which holds psi
+//     and lambda): a thread holds the 2^m amplitudes of one group in registers
+//     (consecutive threads take consecutive groups, so every instruction is a contiguous
+//     run) and applies the m rotations there.
+// At 28 qubits: 5 passes instead of 28 forward, 6 adjoint.
 //
 // Adjoint: the gates of a layer act on different qubits, so each generator G_q commutes
 // with every gate of the layer except its own, and inserting G_q after U_q equals inserting
@@ -32,7 +33,7 @@ namespace {
 
 constexpr int kLB = 256;          // threads per workgroup
 constexpr int kLowQ = 6;          // positions exchanged by lane shuffles in the lane pass (lane bits)
-constexpr int kLowP = 4;          // amplitudes per lane in the lane-shuffle pass
+constexpr int kLowP = 4;          // amplitudes per lane in the lane pass
 constexpr int kQaoaAdjP = 4;      // amplitudes of psi and of lambda per lane in the QAOA adjoint lane pass
 constexpr int kRegQ = 4;          // positions per register pass (adjoint: psi and lambda)
 constexpr int kRegQF = 5;         // positions per register pass (forward)

@@ -376,6 +376,28 @@ __device__ __forceinline__ V quad_row(const Mat4<T>& m, int rw, const V& i0, con
   return r;
 }
 
+// One output row of the 4x4 chosen per lane: row RA, RB, RC or RD for (ul, uh) = (0, 0),
+// (1, 0), (0, 1), (1, 1).  The coefficients are selected (uniform values, per-lane choice)
+// and one row is formed with quad_row's operation order, instead of forming all four rows
+// and keeping one.
+template <int RA, int RB, int RC, int RD, typename T, typename V>
+__device__ __forceinline__ V quad_row_lane(const Mat4<T>& m, bool ul, bool uh, const V& i0, const V& i1, const V& i2,
+                                           const V& i3)
+{
+  Cx<T> c[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const Cx<T> lo = ul ? m.m[4 * RB + k] : m.m[4 * RA + k];
+    const Cx<T> hi = ul ? m.m[4 * RD + k] : m.m[4 * RC + k];
+    c[k] = uh ? hi : lo;
+  }
+  V r = cmad2(c[0], i0, c[1], i1);
+  const V s = cmad2(c[2], i2, c[3], i3);
+  r.x += s.x;
+  r.y += s.y;
+  return r;
+}
+
 // LOB: the lower position is pb (row bit 0); otherwise pa.  Inputs are named by (bit lo,
 // bit hi); rows are row(l, h) = LOB ? 2h + l : 2l + h, all compile-time.
 template <typename T, bool MOVE, bool LOB, int B, int P>
@@ -387,12 +409,6 @@ __device__ __forceinline__ void quad_lane_body(typename Vec<T>::t* __restrict__ 
   const uint64_t r0 = xcd_block(block_id(), grid_blocks()) * (uint64_t)(B * P) + threadIdx.x;
   const int ml = 1 << lo;
   const bool ul = (threadIdx.x >> lo) & 1;  // this lane's bit lo
-  auto rowv = [&](int rw, const V& x00, const V& x10, const V& x01, const V& x11) {
-    // input order by row index
-    const V& i1 = LOB ? x10 : x01;
-    const V& i2 = LOB ? x01 : x10;
-    return quad_row(m, rw, x00, i1, i2, x11);
-  };
   if (hi >= 6) {
     const uint64_t hb = uint64_t(1) << hi;
     V a[P], b[P];
@@ -418,10 +434,10 @@ __device__ __forceinline__ void quad_lane_body(typename Vec<T>::t* __restrict__ 
       } else {
         const V x00 = ul ? pa0 : a[j], x10 = ul ? a[j] : pa0;
         const V x01 = ul ? pb1 : b[j], x11 = ul ? b[j] : pb1;
-        const V y00 = rowv(R00, x00, x10, x01, x11), y10 = rowv(R10, x00, x10, x01, x11);
-        const V y01 = rowv(R01, x00, x10, x01, x11), y11 = rowv(R11, x00, x10, x01, x11);
-        o0 = ul ? y10 : y00;
-        o1 = ul ? y11 : y01;
+        const V& i1 = LOB ? x10 : x01;
+        const V& i2 = LOB ? x01 : x10;
+        o0 = quad_row_lane<R00, R10, R00, R10>(m, ul, false, x00, i1, i2, x11);
+        o1 = quad_row_lane<R01, R11, R01, R11>(m, ul, false, x00, i1, i2, x11);
       }
       stg(o0, &st[idx[j]]);
       stg(o1, &st[idx[j] | hb]);
@@ -454,9 +470,7 @@ __device__ __forceinline__ void quad_lane_body(typename Vec<T>::t* __restrict__ 
         const V x10 = ul ? (uh ? xh : v[j]) : (uh ? xb : xl);
         const V x01 = ul ? (uh ? xl : xb) : (uh ? v[j] : xh);
         const V x11 = ul ? (uh ? v[j] : xh) : (uh ? xl : xb);
-        const V y00 = rowv(R00, x00, x10, x01, x11), y10 = rowv(R10, x00, x10, x01, x11);
-        const V y01 = rowv(R01, x00, x10, x01, x11), y11 = rowv(R11, x00, x10, x01, x11);
-        o = ul ? (uh ? y11 : y10) : (uh ? y01 : y00);
+        o = quad_row_lane<R00, R10, R01, R11>(m, ul, uh, x00, LOB ? x10 : x01, LOB ? x01 : x10, x11);
       }
       stg(o, &st[idx[j]]);
     }

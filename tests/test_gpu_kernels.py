@@ -242,6 +242,35 @@ def test_pauli_expectation_one_pass(dt, paulis):
     assert np.array_equal(sv.download(), h)  # read-only
 
 
+def pauli_apply(a, n, paulis):
+    """P a for a Pauli string, one 2x2 factor per position along its axis of a."""
+    P = {"X": np.array([[0, 1], [1, 0]]), "Y": np.array([[0, -1j], [1j, 0]]), "Z": np.diag([1, -1])}
+    out = a.copy()
+    for q, s in paulis.items():
+        v = out.reshape(1 << (n - 1 - q), 2, 1 << q)
+        out = np.einsum("ij,ajb->aib", P[s], v).reshape(-1)
+    return out
+
+
+@pytest.mark.parametrize("n,paulis", [
+    (4, {0: "X"}), (4, {1: "Y", 3: "X"}), (4, {0: "Y", 2: "Z", 3: "Y"}),
+    (13, {0: "X", 1: "Y", 12: "Z"}), (13, {5: "Y", 2: "X", 7: "Z"}), (13, {6: "X"}),
+    (13, {0: "X", 12: "Y", 3: "Z"}), (13, {11: "Y", 6: "Y", 9: "Z"})])
+def test_pauli_expectation_lane_and_pair_forms(dt, n, paulis):
+    """X/Y positions all below 6 (partner by lane shuffle; below 64 items too) and above."""
+    assert np.allclose(pauli_apply(np.arange(64.0) + 1j, 6, {0: "X", 2: "Y", 5: "Z"}),
+                       pauli_dense(6, {0: "X", 2: "Y", 5: "Z"}) @ (np.arange(64.0) + 1j))
+    h = rand_state(n, dt, 78 + n) * 1.1
+    sv = on_gpu(h, dt)
+    x = sum(1 << q for q, s in paulis.items() if s in "XY")
+    z = sum(1 << q for q, s in paulis.items() if s in "YZ")
+    val, nn = sv.expval_pauli(x, z)
+    a = h.astype(np.complex128)
+    want = np.vdot(a, pauli_apply(a, n, paulis)).real
+    assert abs(val - want) < 1e-12
+    assert abs(nn - np.vdot(a, a).real) < 1e-12
+
+
 def test_scale_axpy_collapse_fill(dt):
     n = 10
     h = rand_state(n, dt, 4)

@@ -1585,24 +1585,26 @@ int do_permute(iqsh_handle h, const unsigned* src_pos)
     PermTile pt;
     std::memset(&pt, 0, sizeof pt);
     pt.m = m;
-    unsigned k = 0, o = 0;
+    unsigned k = 0, o = 0, tb[16];
     for (unsigned b = 0; b < n; ++b) {
       if (in_tile[b]) {
-        pt.tb[k++] = b;
+        tb[k] = b;
+        pt.tj[k++] = uint64_t(1) << b;
       } else {
-        pt.ob[o] = b;
+        pt.oj[o] = uint64_t(1) << b;
         pt.os[o++] = uint64_t(1) << src_pos[b];
       }
     }
     pt.nout = o;
     unsigned order[16];
     for (unsigned i = 0; i < m; ++i) order[i] = i;
-    std::sort(order, order + m, [&](unsigned a, unsigned b) { return src_pos[pt.tb[a]] < src_pos[pt.tb[b]]; });
+    std::sort(order, order + m, [&](unsigned a, unsigned b) { return src_pos[tb[a]] < src_pos[tb[b]]; });
     for (unsigned i = 0; i < m; ++i) {
-      pt.sd[i] = order[i];
-      pt.ss[i] = uint64_t(1) << src_pos[pt.tb[order[i]]];
+      pt.sd[i] = 1u << order[i];
+      pt.ss[i] = uint64_t(1) << src_pos[tb[order[i]]];
     }
-    k_permute_tiled<T><<<grid2(uint64_t(1) << (n - m)), kBlock, 0, g_stream>>>((V*)h->spare, (const V*)h->d, pt);
+    constexpr int TB = perm_threads<T>();
+    k_permute_tiled<T, TB><<<grid2(uint64_t(1) << (n - m)), TB, 0, g_stream>>>((V*)h->spare, (const V*)h->d, pt);
   } else {
     BitTab bt;
     std::memset(&bt, 0, sizeof bt);

@@ -933,72 +933,84 @@ __global__ __launch_bounds__(kBlock) void k_permute(typename Vec<T>::t* __restri
 // (tile bits sorted by source position, so consecutive threads read consecutive source
 // amplitudes), kept in LDS in destination order, and written in destination order.
 // perm_low: 5 bits, runs of 32 amplitudes (512 bytes complex128, 256 complex64).  Measured on
-// MI355X with the loads unrolled: complex64 at 32 qubits 5.65 TB/s (6 bits, 512-byte runs
-// in a 32 KiB tile: 5.40), complex128 at 30 qubits 5.59 TB/s; with one load in flight per
-// thread and iteration the same tiles ran at 1.75 (complex64, 6 bits) and 4.64 TB/s.
+// MI355X with the loads unrolled: complex64 at 32 qubits 5.5-5.65 TB/s (6 bits, 512-byte runs
+// in a 32 KiB tile: 5.40), complex128 at 30 qubits 5.83 TB/s with 512-thread workgroups; with
+// one load in flight per thread and iteration the same tiles ran at 1.75 (complex64, 6
+// bits) and 4.64 TB/s.
 template <typename T>
 constexpr unsigned perm_low() { return 5; }
 struct PermTile {
   unsigned m;          // tile bits
-  unsigned tb[16];     // destination bits of the tile, ascending
-  unsigned sd[16];     // i-th tile bit in source order: its index in tb
+  uint64_t tj[16];     // destination bits of the tile, ascending: 1 << destination position
+  unsigned sd[16];     // i-th tile bit in source order: 1 << its index among the tile bits
   uint64_t ss[16];     // i-th tile bit in source order: 1 << its source position
   unsigned nout;       // destination bits outside the tile
-  unsigned ob[64];     // those bits, ascending (bit i of the tile index sets ob[i])
-  uint64_t os[64];     // 1 << their source positions
+  uint64_t oj[64];     // bit i of the tile index: 1 << its destination position
+  uint64_t os[64];     // bit i of the tile index: 1 << its source position
 };
 
+// Threads per workgroup of k_permute_tiled: 512 complex128 (two amplitudes per thread; 256
+// threads with four ran 4% slower at 30 qubits), 256 complex64 (four; 512 with two ran 15%
+// slower at 32 qubits).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_permute_tiled(typename Vec<T>::t* __restrict__ dst,
-                                                          const typename Vec<T>::t* __restrict__ src, PermTile pt)
+constexpr int perm_threads() { return sizeof(T) == 8 ? 512 : 256; }
+
+template <typename T, int TB>
+__global__ __launch_bounds__(TB) void k_permute_tiled(typename Vec<T>::t* __restrict__ dst,
+                                                      const typename Vec<T>::t* __restrict__ src, PermTile pt)
 {
   using V = typename Vec<T>::t;
-  // The host pads every tile to exactly LT bits: each thread moves M / kBlock amplitudes, all
-  // of its loads in flight at once.  Tile index r = threadIdx.x + k kBlock: its low 8 bits
-  // come from the thread, the rest from k (a constant in the unrolled loops).
-  constexpr unsigned LT = 2 * perm_low<T>(), M = 1u << LT, LB = 8, K = M / kBlock;
-  static_assert(kBlock == (1 << LB) && M >= kBlock, "tile of at least one amplitude per thread");
+  // The host pads every tile to exactly LT bits: each of the TB threads moves M / TB
+  // amplitudes, all of its loads in flight at once.  Tile position r = threadIdx.x + k TB:
+  // its low LB bits come from the thread, the rest from k (a constant in the unrolled
+  // loops).  The per-thread offsets are selected without branches, so that their scalar
+  // loads issue together instead of one wait per set bit.
+  constexpr unsigned LT = 2 * perm_low<T>(), M = 1u << LT, LB = TB == 512 ? 9 : 8, K = M / TB;
+  static_assert(TB == (1 << LB) && M >= TB, "tile of at least one amplitude per thread");
   __shared__ V tile[M];
   const uint64_t t = block_id();
-  uint64_t j0 = 0, s0 = 0;
-  for (unsigned i = 0; i < pt.nout; ++i)
-    if ((t >> i) & 1) {
-      j0 |= uint64_t(1) << pt.ob[i];
-      s0 |= pt.os[i];
-    }
-  uint64_t sa_t = s0, ja_t = j0;
+  uint64_t sa = 0, ja = 0;
   unsigned d_t = 0;
+#pragma unroll 4
+  for (unsigned i = 0; i < pt.nout; ++i) {
+    const bool on = (t >> i) & 1;
+    const uint64_t oj = pt.oj[i], os = pt.os[i];
+    ja |= on ? oj : 0;
+    sa |= on ? os : 0;
+  }
 #pragma unroll
-  for (unsigned i = 0; i < LB; ++i)
-    if ((threadIdx.x >> i) & 1) {
-      sa_t |= pt.ss[i];
-      d_t |= 1u << pt.sd[i];
-      ja_t |= uint64_t(1) << pt.tb[i];
-    }
+  for (unsigned i = 0; i < LB; ++i) {
+    const bool on = (threadIdx.x >> i) & 1;
+    const uint64_t ss = pt.ss[i], tj = pt.tj[i];
+    const unsigned sd = pt.sd[i];
+    sa |= on ? ss : 0;
+    ja |= on ? tj : 0;
+    d_t |= on ? sd : 0;
+  }
   V v[K];
   unsigned d[K];
 #pragma unroll
   for (unsigned k = 0; k < K; ++k) {
-    uint64_t sa = sa_t;
+    uint64_t s = sa;
     d[k] = d_t;
 #pragma unroll
     for (unsigned i = LB; i < LT; ++i)
       if ((k >> (i - LB)) & 1) {
-        sa |= pt.ss[i];
-        d[k] |= 1u << pt.sd[i];
+        s |= pt.ss[i];
+        d[k] |= pt.sd[i];
       }
-    v[k] = ldg(&src[sa]);
+    v[k] = ldg(&src[s]);
   }
 #pragma unroll
   for (unsigned k = 0; k < K; ++k) tile[d[k]] = v[k];
   __syncthreads();
 #pragma unroll
   for (unsigned k = 0; k < K; ++k) {
-    uint64_t ja = ja_t;
+    uint64_t j = ja;
 #pragma unroll
     for (unsigned i = LB; i < LT; ++i)
-      if ((k >> (i - LB)) & 1) ja |= uint64_t(1) << pt.tb[i];
-    stg(tile[threadIdx.x + k * kBlock], &dst[ja]);
+      if ((k >> (i - LB)) & 1) j |= pt.tj[i];
+    stg(tile[threadIdx.x + k * TB], &dst[j]);
   }
 }
 

@@ -271,6 +271,28 @@ def test_pauli_expectation_lane_and_pair_forms(dt, n, paulis):
     assert abs(nn - np.vdot(a, a).real) < 1e-12
 
 
+@pytest.mark.parametrize("n", [9, 20])
+def test_reductions_repeat_bit_identically(dt, n):
+    """The summation order depends on the size only: repeated reductions, with other
+    reductions (other grids) between them, return the same bits.  n = 9 leaves every thread
+    in the tail of its first chunk; n = 20 runs whole chunks on many workgroups."""
+    h = rand_state(n, dt, 90 + n)
+    g = rand_state(n, dt, 91 + n)
+    sv, sg = on_gpu(h, dt), on_gpu(g, dt)
+
+    def everything():
+        return ([sv.norm2()] + [sv.prob1(p) for p in (0, 3, n - 1)]
+                + [v for xz in ((0, 5), (3, 6), (1 << (n - 1), 1), ((1 << 7) | 2, 4)) for v in sv.expval_pauli(*xz)]
+                + [sv.overlap(sg).real, sv.overlap(sg).imag, sv.maxabsdiff(sg, 1.0)])
+
+    first = everything()
+    sg.norm2()
+    second = everything()
+    assert [float(x).hex() for x in first] == [float(x).hex() for x in second]
+    a = h.astype(np.complex128)
+    assert abs(first[0] - np.vdot(a, a).real) < 1e-12
+
+
 def test_scale_axpy_collapse_fill(dt):
     n = 10
     h = rand_state(n, dt, 4)

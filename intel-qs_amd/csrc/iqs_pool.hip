@@ -27,6 +27,7 @@ namespace {
 constexpr int kPB = 256;              // threads per workgroup
 constexpr unsigned kChunkLog = 14;    // at most 2^14 amplitudes per reduction workgroup
 constexpr uint64_t kMaxGrid = 1u << 20;
+constexpr int kRU = 8;                // loads in flight per thread in the chunk reductions
 
 // Device buffers shared by the pool entry points (guarded; grown on demand).
 struct PoolWork {
@@ -119,6 +120,120 @@ __global__ __launch_bounds__(kPB) void k_pool_1q(typename Vec<T>::t* __restrict_
   }
 }
 
+// The same per-state updates in the form of the register's own gate kernels: exactly one
+// tile of B * P items per workgroup, tiles XCD-contiguous, both items of a thread loaded
+// before either is updated, non-temporal accesses when an item's amplitudes lie in
+// different 128-byte lines (cached when they share lines, so the second access finds the
+// line in L2).  Items of states at or beyond nstates are neither read nor written.
+constexpr int kTB = 128, kTP = 2;
+__host__ __device__ constexpr unsigned pool_line_bits(int esz) { return esz == 16 ? 3u : 4u; }
+
+__device__ __forceinline__ uint64_t pool_tile() {
+  const uint64_t b = blockIdx.x + (uint64_t)blockIdx.y * gridDim.x, G = (uint64_t)gridDim.x * gridDim.y;
+  return (G & 7) ? b : xcd_block(b, G);
+}
+
+template <typename T, bool NT>
+__global__ __launch_bounds__(kTB) void k_pool_1q_t(typename Vec<T>::t* __restrict__ st, unsigned pos, unsigned sbits,
+                                                   const Mat2<T>* __restrict__ mats, uint64_t nstates)
+{
+  using V = typename Vec<T>::t;
+  const uint64_t hb = uint64_t(1) << pos, k0 = pool_tile() * (kTB * kTP) + threadIdx.x;
+  V a[kTP], b[kTP];
+  uint64_t i0[kTP];
+  bool ok[kTP];
+#pragma unroll
+  for (int j = 0; j < kTP; ++j) {
+    i0[j] = ins0(k0 + (uint64_t)j * kTB, pos);
+    ok[j] = (i0[j] >> sbits) < nstates;
+    if (ok[j]) {
+      a[j] = ldx<NT>(&st[i0[j]]);
+      b[j] = ldx<NT>(&st[i0[j] | hb]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kTP; ++j) {
+    if (!ok[j]) continue;
+    const Mat2<T> m = mats[i0[j] >> sbits];
+    stx<NT>(cmad2(m.m00, a[j], m.m01, b[j]), &st[i0[j]]);
+    stx<NT>(cmad2(m.m10, a[j], m.m11, b[j]), &st[i0[j] | hb]);
+  }
+}
+
+// Per-state 2x2 on a position inside a 128-byte line, in whole runs: thread t of tile b
+// holds amplitude i = b * kLB * kLP + j * kLB + t, consecutive lanes consecutive amplitudes,
+// and receives its partner i ^ 2^pos (same wavefront, same state) by __shfl_xor.  States of
+// at least kLB * kLP amplitudes (the host's condition), so a tile lies in one state and its
+// matrix is one uniform load; a tile of a state at or beyond nstates is neither read nor
+// written.
+constexpr int kLB = 256, kLP = 2;
+constexpr unsigned kLaneMinBits = 9;  // 2^9 = kLB * kLP
+template <typename T>
+__global__ __launch_bounds__(kLB) void k_pool_1q_lane(typename Vec<T>::t* __restrict__ st, unsigned pos,
+                                                      unsigned sbits, const Mat2<T>* __restrict__ mats,
+                                                      uint64_t nstates)
+{
+  using V = typename Vec<T>::t;
+  const uint64_t t0 = pool_tile() * (kLB * kLP), s = t0 >> sbits;
+  if (s >= nstates) return;  // the whole tile: uniform
+  const Mat2<T> m = mats[s];
+  const uint64_t i0 = t0 + threadIdx.x;
+  const int ml = 1 << pos;
+  const bool up = (threadIdx.x >> pos) & 1;  // this amplitude's bit pos
+  V a[kLP];
+#pragma unroll
+  for (int j = 0; j < kLP; ++j) a[j] = ldg(&st[i0 + (uint64_t)j * kLB]);
+#pragma unroll
+  for (int j = 0; j < kLP; ++j) {
+    V b;  // the partner amplitude
+    b.x = __shfl_xor(a[j].x, ml);
+    b.y = __shfl_xor(a[j].y, ml);
+    stg(up ? cmad2(m.m10, b, m.m11, a[j]) : cmad2(m.m00, a[j], m.m01, b), &st[i0 + (uint64_t)j * kLB]);
+  }
+}
+
+template <typename T, bool NT>
+__global__ __launch_bounds__(kTB) void k_pool_2q_t(typename Vec<T>::t* __restrict__ st, unsigned lo, unsigned hi,
+                                                   uint64_t dh, uint64_t dl, unsigned sbits,
+                                                   const Mat4<T>* __restrict__ mats, uint64_t nstates)
+{
+  using V = typename Vec<T>::t;
+  const uint64_t k0 = pool_tile() * (kTB * kTP) + threadIdx.x;
+  const uint64_t off[4] = {0, dl, dh, dh | dl};
+  V x[kTP][4];
+  uint64_t i0[kTP];
+  bool ok[kTP];
+#pragma unroll
+  for (int j = 0; j < kTP; ++j) {
+    i0[j] = ins0(ins0(k0 + (uint64_t)j * kTB, lo), hi);
+    ok[j] = (i0[j] >> sbits) < nstates;
+    if (ok[j]) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) x[j][c] = ldx<NT>(&st[i0[j] | off[c]]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kTP; ++j) {
+    if (!ok[j]) continue;
+    const Mat4<T>& m = mats[i0[j] >> sbits];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      V r = cmad2(m.m[4 * t + 0], x[j][0], m.m[4 * t + 1], x[j][1]);
+      const V u = cmad2(m.m[4 * t + 2], x[j][2], m.m[4 * t + 3], x[j][3]);
+      r.x += u.x;
+      r.y += u.y;
+      stx<NT>(r, &st[i0[j] | off[t]]);
+    }
+  }
+}
+
+// 2D grid of `tiles` workgroups (at most 65536 in x).
+dim3 tile_grid(uint64_t tiles)
+{
+  const uint64_t gx = tiles < 65536 ? tiles : 65536;
+  return dim3((unsigned)gx, (unsigned)((tiles + gx - 1) / gx), 1);
+}
+
 // Dense 4x4 per state on (ph, pl), rows 2 b(ph) + b(pl): with lo < hi the two positions in
 // ascending order, item k enumerates the indices with both bits clear; s = index >> sbits.
 template <typename T>
@@ -155,12 +270,22 @@ __global__ __launch_bounds__(kPB) void k_pool_red(const typename Vec<T>::t* __re
 {
   const uint64_t base = (uint64_t)blockIdx.x << clog, C = uint64_t(1) << clog;
   double v[1] = {0};
-  for (uint64_t j = threadIdx.x; j < C; j += kPB) {
-    const uint64_t i = base + j;
-    const double p = nrm2(st[i]);
+  auto add = [&](uint64_t i, double p) {
     if (KIND == 0) v[0] += p;
     else if (KIND == 1) v[0] += ((i >> arg) & 1) ? p : 0.0;
     else v[0] += (__popcll(i & arg) & 1) ? -p : p;
+  };
+  if (C >= kRU * kPB) {
+    // C is a power of two: whole batches of kRU loads per thread, in flight together
+    for (uint64_t j = threadIdx.x; j < C; j += kRU * kPB) {
+      typename Vec<T>::t a[kRU];
+#pragma unroll
+      for (int u = 0; u < kRU; ++u) a[u] = ldg(&st[base + j + u * kPB]);
+#pragma unroll
+      for (int u = 0; u < kRU; ++u) add(base + j + u * kPB, nrm2(a[u]));
+    }
+  } else {
+    for (uint64_t j = threadIdx.x; j < C; j += kPB) add(base + j, nrm2(ldg(&st[base + j])));
   }
   wg_sum<1>(v, part + blockIdx.x);
 }
@@ -174,11 +299,25 @@ __global__ __launch_bounds__(kPB) void k_pool_ovl(const typename Vec<T>::t* __re
   const uint64_t base = (uint64_t)blockIdx.x << clog, C = uint64_t(1) << clog;
   const uint64_t smask = (uint64_t(1) << sbits) - 1;
   double v[2] = {0, 0};
-  for (uint64_t j = threadIdx.x; j < C; j += kPB) {
-    const uint64_t i = base + j;
-    const auto a = st[i], b = ref[i & smask];
+  using V = typename Vec<T>::t;
+  auto add = [&](const V& a, const V& b) {
     v[0] += (double)b.x * a.x + (double)b.y * a.y;
     v[1] += (double)b.x * a.y - (double)b.y * a.x;
+  };
+  if (C >= kRU * kPB) {
+    for (uint64_t j = threadIdx.x; j < C; j += kRU * kPB) {
+      V a[kRU], b[kRU];
+#pragma unroll
+      for (int u = 0; u < kRU; ++u) {
+        const uint64_t i = base + j + u * kPB;
+        a[u] = ldg(&st[i]);
+        b[u] = ref[i & smask];
+      }
+#pragma unroll
+      for (int u = 0; u < kRU; ++u) add(a[u], b[u]);
+    }
+  } else {
+    for (uint64_t j = threadIdx.x; j < C; j += kPB) add(ldg(&st[base + j]), ref[(base + j) & smask]);
   }
   double out[2];
   wg_sum<2>(v, out);
@@ -221,8 +360,21 @@ int do_1q_each(iqsh_handle h, unsigned sbits, unsigned pos, const double* mats, 
   IQSI_TRY(hipStreamSynchronize(iqsi::stream()));
   IQSI_TRY(hipMemcpy(g_pw.mats, m.data(), bytes, hipMemcpyHostToDevice));
   const uint64_t items = h->size() >> 1;
-  k_pool_1q<T><<<grid_of(items), kPB, 0, iqsi::stream()>>>((typename Vec<T>::t*)h->d, items, pos, sbits,
-                                                            (const Mat2<T>*)g_pw.mats, nstates);
+  using V = typename Vec<T>::t;
+  if (pos < pool_line_bits(sizeof(V)) && sbits >= kLaneMinBits) {
+    k_pool_1q_lane<T><<<tile_grid(h->size() / (kLB * kLP)), kLB, 0, iqsi::stream()>>>(
+        (V*)h->d, pos, sbits, (const Mat2<T>*)g_pw.mats, nstates);
+  } else if (items % (kTB * kTP) == 0) {
+    // items is a power of two: a whole number of tiles, 8 or more from 2^12 amplitudes up
+    const dim3 g = tile_grid(items / (kTB * kTP));
+    if (pos >= pool_line_bits(sizeof(V)))
+      k_pool_1q_t<T, true><<<g, kTB, 0, iqsi::stream()>>>((V*)h->d, pos, sbits, (const Mat2<T>*)g_pw.mats, nstates);
+    else
+      k_pool_1q_t<T, false><<<g, kTB, 0, iqsi::stream()>>>((V*)h->d, pos, sbits, (const Mat2<T>*)g_pw.mats, nstates);
+  } else {
+    k_pool_1q<T><<<grid_of(items), kPB, 0, iqsi::stream()>>>((V*)h->d, items, pos, sbits, (const Mat2<T>*)g_pw.mats,
+                                                              nstates);
+  }
   IQSI_TRY(hipGetLastError());
   return IQSH_OK;
 }
@@ -238,9 +390,21 @@ int do_2q_each(iqsh_handle h, unsigned sbits, unsigned ph, unsigned pl, const do
   IQSI_TRY(hipStreamSynchronize(iqsi::stream()));
   IQSI_TRY(hipMemcpy(g_pw.mats, m.data(), bytes, hipMemcpyHostToDevice));
   const uint64_t items = h->size() >> 2;
-  k_pool_2q<T><<<grid_of(items), kPB, 0, iqsi::stream()>>>((typename Vec<T>::t*)h->d, items, std::min(ph, pl),
-                                                            std::max(ph, pl), uint64_t(1) << ph, uint64_t(1) << pl,
-                                                            sbits, (const Mat4<T>*)g_pw.mats, nstates);
+  using V = typename Vec<T>::t;
+  const unsigned lo = std::min(ph, pl), hi = std::max(ph, pl);
+  if (items % (kTB * kTP) == 0) {
+    const dim3 g = tile_grid(items / (kTB * kTP));
+    if (lo >= pool_line_bits(sizeof(V)))
+      k_pool_2q_t<T, true><<<g, kTB, 0, iqsi::stream()>>>((V*)h->d, lo, hi, uint64_t(1) << ph, uint64_t(1) << pl, sbits,
+                                                          (const Mat4<T>*)g_pw.mats, nstates);
+    else
+      k_pool_2q_t<T, false><<<g, kTB, 0, iqsi::stream()>>>((V*)h->d, lo, hi, uint64_t(1) << ph, uint64_t(1) << pl,
+                                                           sbits, (const Mat4<T>*)g_pw.mats, nstates);
+  } else {
+    k_pool_2q<T><<<grid_of(items), kPB, 0, iqsi::stream()>>>((V*)h->d, items, lo, hi, uint64_t(1) << ph,
+                                                              uint64_t(1) << pl, sbits, (const Mat4<T>*)g_pw.mats,
+                                                              nstates);
+  }
   IQSI_TRY(hipGetLastError());
   return IQSH_OK;
 }

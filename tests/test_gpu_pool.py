@@ -197,3 +197,67 @@ def test_pool_errors():
         pool.Initialize("rand", 0)
     with pytest.raises(ValueError):
         m.MPIEnvironment.UpdateStateComm(4)
+
+
+@pytest.mark.parametrize("dt", ["c128", "c64"])
+def test_per_state_kernels_every_form(dt):
+    """iqsh_pool_* on 9-qubit states (3 states in 4 slots, the fourth left as uploaded):
+    per-state 1q at positions below 6 (lane-shuffle form) and from 6 up (pair tiles), per-state
+    2q with both positions inside a 128-byte line and outside it, then every per-state
+    reduction, each against numpy on the same states."""
+    import ctypes as C
+
+    from iqs_amd._lib import call
+    from iqs_amd.circuits import SplitMix64, haar_unitary
+    from iqs_amd.engine import StateVector
+
+    sq, ns, n = 9, 3, 11
+    npdt = np.complex128 if dt == "c128" else np.complex64
+    tol = 1e-12 if dt == "c128" else 2e-5
+    r = np.random.default_rng(61)
+    host = (r.normal(size=1 << n) + 1j * r.normal(size=1 << n)).astype(npdt)
+    sv = StateVector(n, dt)
+    sv.upload(host)
+    want = host.astype(np.complex128).reshape(1 << (n - sq), 1 << sq)
+    rng = SplitMix64(62)
+
+    def mats(k):
+        us = [haar_unitary(1 << k, rng) for _ in range(ns)]
+        flat = np.concatenate([np.stack([u.real, u.imag], -1).reshape(-1) for u in us]).astype(np.float64)
+        return us, flat
+
+    for pos in (0, 3, 5, 6, 8):
+        us, flat = mats(1)
+        call("iqsh_pool_apply_1q_each", sv.handle, sq, pos, flat.ctypes.data_as(C.POINTER(C.c_double)), ns)
+        for s in range(ns):
+            v = want[s].reshape(1 << (sq - 1 - pos), 2, 1 << pos)
+            want[s] = np.einsum("ij,ajb->aib", us[s].astype(npdt).astype(np.complex128), v).reshape(-1)
+    idx = np.arange(1 << sq)
+    for ph, pl in ((1, 0), (2, 7), (8, 6)):
+        us, flat = mats(2)
+        call("iqsh_pool_apply_2q_each", sv.handle, sq, ph, pl, flat.ctypes.data_as(C.POINTER(C.c_double)), ns)
+        base = idx[((idx >> ph) & 1) == 0]
+        base = base[((base >> pl) & 1) == 0]
+        rows = [base, base | (1 << pl), base | (1 << ph), base | (1 << ph) | (1 << pl)]
+        for s in range(ns):
+            x = np.stack([want[s][rw] for rw in rows])
+            y = us[s].astype(npdt).astype(np.complex128) @ x
+            for t, rw in enumerate(rows):
+                want[s][rw] = y[t]
+    got = sv.download().astype(np.complex128).reshape(1 << (n - sq), 1 << sq)
+    assert np.abs(got[:ns] - want[:ns]).max() < tol
+    assert np.array_equal(got[ns:], host.astype(np.complex128).reshape(1 << (n - sq), 1 << sq)[ns:])
+    out = (C.c_double * (2 * ns))()
+    for kind, arg, f in ((0, 0, lambda v: np.sum(np.abs(v) ** 2)),
+                         (1, 4, lambda v: np.sum(np.abs(v[(idx >> 4) & 1 == 1]) ** 2)),
+                         (2, 0b100010001, lambda v: np.sum(np.where(
+                             np.array([bin(i & 0b100010001).count("1") & 1 for i in idx]) == 1, -1.0, 1.0)
+                             * np.abs(v) ** 2))):
+        call("iqsh_pool_reduce", sv.handle, sq, kind, arg, out, ns)
+        assert np.max(np.abs(np.array(out[:ns]) - [f(got[s]) for s in range(ns)])) < 1e-11
+    ref_h = (r.normal(size=1 << sq) + 1j * r.normal(size=1 << sq)).astype(npdt)
+    ref = StateVector(sq, dt)
+    ref.upload(ref_h)
+    call("iqsh_pool_overlap", sv.handle, sq, ref.handle, out, ns)
+    wov = [np.vdot(ref_h.astype(np.complex128), got[s]) for s in range(ns)]
+    assert np.max(np.abs(np.array(out[0::2]) + 1j * np.array(out[1::2]) - wov)) < 1e-11

@@ -172,6 +172,59 @@ def kind_sweep(sv, n: int, esz: int, reps: int, stream, torch):
     return out
 
 
+def pool_sweep(reps: int, stream, torch):
+    """Pool of states (the reference's pool, src/mpi_env.cpp:239-404, as one register whose
+    top positions index the states): 8 states of 27 qubits complex128 and 8 of 28 complex64
+    (16 GiB each), per-state 1q (noise gates, 1q channel branches) at low, middle and top
+    positions and per-state 2q (2q channel branches), against 2S; per-state norm, P(qubit)
+    and Z-string sums against S; per-state overlap with a reference state against 2S (the
+    pool and the reference once per state).  Every call includes its host work: the matrix
+    upload of the per-state gates, the per-state sums of the reductions."""
+    import ctypes as C
+
+    import numpy as np
+    from iqs_amd._lib import call
+    from iqs_amd.circuits import SplitMix64, haar_unitary
+    from iqs_amd.engine import StateVector
+
+    out = {}
+    for dt, sq in (("c128", 27), ("c64", 28)):
+        ns = 8
+        n = sq + 3
+        S = (1 << n) * (16 if dt == "c128" else 8)
+        sv = StateVector(n, dt)
+        call("iqsh_pool_init", sv.handle, sq, 1, 0, ns)
+        rng = SplitMix64(20240605)
+
+        def mats(k):
+            return np.concatenate([np.stack([u.real, u.imag], -1).reshape(-1) for u in
+                                   (haar_unitary(1 << k, rng) for _ in range(ns))]).astype(np.float64)
+
+        m2, m4 = mats(1), mats(2)
+        p2, p4 = (m.ctypes.data_as(C.POINTER(C.c_double)) for m in (m2, m4))
+        res = {}
+        rows = [((p,), _time_ms(lambda: call("iqsh_pool_apply_1q_each", sv.handle, sq, p, p2, ns), reps, stream, torch))
+                for p in (0, 2, 5, 13, sq - 1)]
+        res["per_state_1q"] = dict(_rates(rows, 2 * S), cases=[list(t) for t, _ in rows])
+        rows = [((a, b), _time_ms(lambda: call("iqsh_pool_apply_2q_each", sv.handle, sq, a, b, p4, ns), reps, stream,
+                                  torch)) for a, b in ((1, 0), (5, 20), (sq - 1, sq - 2))]
+        res["per_state_2q"] = dict(_rates(rows, 2 * S), cases=[list(t) for t, _ in rows])
+        red = (C.c_double * (2 * ns))()
+        rows = [(nm, _time_ms(lambda: call("iqsh_pool_reduce", sv.handle, sq, kind, arg, red, ns), reps, stream, torch))
+                for nm, kind, arg in (("norm", 0, 0), ("prob1_3", 1, 3), ("zstring", 2, (1 << 5) | 1))]
+        res["per_state_reductions"] = dict(_rates(rows, S), cases=[t for t, _ in rows])
+        ref = StateVector(sq, dt)
+        ref.init_basis(0)
+        ms = _time_ms(lambda: call("iqsh_pool_overlap", sv.handle, sq, ref.handle, red, ns), reps, stream, torch)
+        res["per_state_overlap_GBs"] = 2 * S / ms / 1e6
+        ref.close()
+        sv.close()
+        res.update({"states": ns, "state_qubits": sq})
+        out[dt] = res
+        log(f"sweep pool ({ns} x {sq} qubits, {dt}): per-state 1q {res['per_state_1q']['avg_GBs']:.0f} GB/s average")
+    return out
+
+
 def permute_sweep(sv, n: int, esz: int, reps: int, stream, torch):
     """PermuteQubits (src/qureg_permute.cpp:57-105) as one bit-exact index permutation:
     reads and writes the state once (2S).  Each case is applied and then undone, so the
@@ -832,6 +885,7 @@ def main() -> int:
         sweep_c64["reductions"] = reduction_sweep(sv64, n64, 8, args.sweep_reps, stream, torch)
         sweep_c64["qubits"] = n64
         sv64.close()
+    pool = None if args.no_sweep else pool_sweep(args.sweep_reps, stream, torch)
     log("brick circuit and per-kind sweep done")
     qaoa = None if args.no_qaoa else qaoa_measure(args.qaoa_qubits, 8, not args.qaoa_sample, torch)
     log("QAOA gradient done")
@@ -882,6 +936,7 @@ def main() -> int:
         "sweep_permute": perm,
         "sweep_reductions": red,
         "sweep_c64": sweep_c64,
+        "sweep_pool": pool,
         "qaoa_grad": qaoa,
         "fusion": fusion,
         "hea_batch": batch,

@@ -192,6 +192,49 @@ __global__ __launch_bounds__(kLB) void k_pool_1q_lane(typename Vec<T>::t* __rest
   }
 }
 
+// Per-state 4x4 on (ph, pl), both inside a 128-byte line, in whole runs as k_pool_1q_lane:
+// a lane holds the amplitude whose row is r = 2 b(ph) + b(pl) and receives rows r ^ 1,
+// r ^ 2, r ^ 3 from lanes ^ 2^pl, ^ 2^ph, ^ both; its output is row r of the state's
+// matrix, the coefficients m[r][r ^ k] selected per lane from the uniform matrix.
+template <typename T>
+__global__ __launch_bounds__(kLB) void k_pool_2q_lane(typename Vec<T>::t* __restrict__ st, unsigned ph, unsigned pl,
+                                                      unsigned sbits, const Mat4<T>* __restrict__ mats,
+                                                      uint64_t nstates)
+{
+  using V = typename Vec<T>::t;
+  const uint64_t t0 = pool_tile() * (kLB * kLP), s = t0 >> sbits;
+  if (s >= nstates) return;  // the whole tile: uniform
+  // The state's matrix in LDS: per-lane coefficient choices from global memory compiled to
+  // per-lane gathers (complex64 (1,0): 4.0 TB/s).
+  __shared__ Cx<T> sm[16];
+  if (threadIdx.x < 16) sm[threadIdx.x] = mats[s].m[threadIdx.x];
+  __syncthreads();
+  const int mh = 1 << ph, ml = 1 << pl;
+  const int r = (((threadIdx.x >> ph) & 1) << 1) | ((threadIdx.x >> pl) & 1);
+  Cx<T> c[4];  // c[k] = m[r][r ^ k]
+#pragma unroll
+  for (int k = 0; k < 4; ++k) c[k] = sm[4 * r + (r ^ k)];
+  const uint64_t i0 = t0 + threadIdx.x;
+  V a[kLP];
+#pragma unroll
+  for (int j = 0; j < kLP; ++j) a[j] = ldg(&st[i0 + (uint64_t)j * kLB]);
+#pragma unroll
+  for (int j = 0; j < kLP; ++j) {
+    V xl, xh, xb;
+    xl.x = __shfl_xor(a[j].x, ml);
+    xl.y = __shfl_xor(a[j].y, ml);
+    xh.x = __shfl_xor(a[j].x, mh);
+    xh.y = __shfl_xor(a[j].y, mh);
+    xb.x = __shfl_xor(a[j].x, ml | mh);
+    xb.y = __shfl_xor(a[j].y, ml | mh);
+    V r = cmad2(c[0], a[j], c[1], xl);
+    const V u = cmad2(c[2], xh, c[3], xb);
+    r.x += u.x;
+    r.y += u.y;
+    stg(r, &st[i0 + (uint64_t)j * kLB]);
+  }
+}
+
 template <typename T, bool NT>
 __global__ __launch_bounds__(kTB) void k_pool_2q_t(typename Vec<T>::t* __restrict__ st, unsigned lo, unsigned hi,
                                                    uint64_t dh, uint64_t dl, unsigned sbits,
@@ -392,7 +435,10 @@ int do_2q_each(iqsh_handle h, unsigned sbits, unsigned ph, unsigned pl, const do
   const uint64_t items = h->size() >> 2;
   using V = typename Vec<T>::t;
   const unsigned lo = std::min(ph, pl), hi = std::max(ph, pl);
-  if (items % (kTB * kTP) == 0) {
+  if (hi < pool_line_bits(sizeof(V)) && sbits >= kLaneMinBits) {
+    k_pool_2q_lane<T><<<tile_grid(h->size() / (kLB * kLP)), kLB, 0, iqsi::stream()>>>(
+        (V*)h->d, ph, pl, sbits, (const Mat4<T>*)g_pw.mats, nstates);
+  } else if (items % (kTB * kTP) == 0) {
     const dim3 g = tile_grid(items / (kTB * kTP));
     if (lo >= pool_line_bits(sizeof(V)))
       k_pool_2q_t<T, true><<<g, kTB, 0, iqsi::stream()>>>((V*)h->d, lo, hi, uint64_t(1) << ph, uint64_t(1) << pl, sbits,

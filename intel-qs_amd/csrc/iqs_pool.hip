@@ -125,7 +125,7 @@ __global__ __launch_bounds__(kPB) void k_pool_1q(typename Vec<T>::t* __restrict_
 // before either is updated, non-temporal accesses when an item's amplitudes lie in
 // different 128-byte lines (cached when they share lines, so the second access finds the
 // line in L2).  Items of states at or beyond nstates are neither read nor written.
-constexpr int kTB = 128, kTP = 2;
+constexpr int kTB = 128, kTP = 2, kTP2 = 1;
 __host__ __device__ constexpr unsigned pool_line_bits(int esz) { return esz == 16 ? 3u : 4u; }
 
 __device__ __forceinline__ uint64_t pool_tile() {
@@ -241,13 +241,13 @@ __global__ __launch_bounds__(kTB) void k_pool_2q_t(typename Vec<T>::t* __restric
                                                    const Mat4<T>* __restrict__ mats, uint64_t nstates)
 {
   using V = typename Vec<T>::t;
-  const uint64_t k0 = pool_tile() * (kTB * kTP) + threadIdx.x;
+  const uint64_t k0 = pool_tile() * (kTB * kTP2) + threadIdx.x;
   const uint64_t off[4] = {0, dl, dh, dh | dl};
-  V x[kTP][4];
-  uint64_t i0[kTP];
-  bool ok[kTP];
+  V x[kTP2][4];
+  uint64_t i0[kTP2];
+  bool ok[kTP2];
 #pragma unroll
-  for (int j = 0; j < kTP; ++j) {
+  for (int j = 0; j < kTP2; ++j) {
     i0[j] = ins0(ins0(k0 + (uint64_t)j * kTB, lo), hi);
     ok[j] = (i0[j] >> sbits) < nstates;
     if (ok[j]) {
@@ -256,7 +256,7 @@ __global__ __launch_bounds__(kTB) void k_pool_2q_t(typename Vec<T>::t* __restric
     }
   }
 #pragma unroll
-  for (int j = 0; j < kTP; ++j) {
+  for (int j = 0; j < kTP2; ++j) {
     if (!ok[j]) continue;
     const Mat4<T>& m = mats[i0[j] >> sbits];
 #pragma unroll
@@ -438,8 +438,8 @@ int do_2q_each(iqsh_handle h, unsigned sbits, unsigned ph, unsigned pl, const do
   if (hi < pool_line_bits(sizeof(V)) && sbits >= kLaneMinBits) {
     k_pool_2q_lane<T><<<tile_grid(h->size() / (kLB * kLP)), kLB, 0, iqsi::stream()>>>(
         (V*)h->d, ph, pl, sbits, (const Mat4<T>*)g_pw.mats, nstates);
-  } else if (items % (kTB * kTP) == 0) {
-    const dim3 g = tile_grid(items / (kTB * kTP));
+  } else if (items % (kTB * kTP2) == 0) {
+    const dim3 g = tile_grid(items / (kTB * kTP2));
     if (lo >= pool_line_bits(sizeof(V)))
       k_pool_2q_t<T, true><<<g, kTB, 0, iqsi::stream()>>>((V*)h->d, lo, hi, uint64_t(1) << ph, uint64_t(1) << pl, sbits,
                                                           (const Mat4<T>*)g_pw.mats, nstates);

@@ -162,8 +162,11 @@ constexpr int kTB = 1024, kTP = 2;  // pair and 4x4 kernels for complex64 (tiled
 // runs, one wavefront access each): dense 1q 6.15 -> 6.53 TB/s for targets >= 11, controlled
 // 5.88 -> 6.17, dense 2q 5.80 -> 6.18; complex64 keeps 1024 x 2 (smaller blocks are no
 // faster there).
-template <typename T> struct PairTile { static constexpr int B = 1024, P = 2, QP = 1; };
-template <> struct PairTile<double> { static constexpr int B = 128, P = 2, QP = 2; };
+// QB x QP: the dense 4x4 kernel with a position above 10 (256 x 1 for both precisions:
+// 30q complex128 at (5,20), (8,25), (12,13), (20,21), (24,25), (28,29), (15,29) averaged
+// 6.14 -> 6.27 TB/s against 128 x 2, 32q complex64 6.16 -> 6.30 against 1024 x 1).
+template <typename T> struct PairTile { static constexpr int B = 1024, P = 2, QB = 256, QP = 1; };
+template <> struct PairTile<double> { static constexpr int B = 128, P = 2, QB = 256, QP = 1; };
 constexpr int kLB = 256, kLP = 2;   // lane-shuffle 1q kernel
 // Lane-shuffle 1q kernel and 2q diagonal, per precision (30q sweeps): complex128 lane kernel
 // 64 x 2 (one wavefront per workgroup) 6.36 -> 6.60 TB/s against 256 x 2, complex64 keeps
@@ -1368,7 +1371,7 @@ int do_2q(iqsh_handle h, unsigned ph, unsigned pl, const double* m)
   if (std::min(ph, pl) < line_bits<T>() && h->size() >= (kTiledMin << 3)) return launch_quad_lane<T>(h, ph, pl, M, false);
   const unsigned ins[2] = {ph, pl};
   const uint64_t items = h->size() >> 2;
-  constexpr int TB = PairTile<T>::B, TP = PairTile<T>::QP;
+  constexpr int TB = PairTile<T>::QB, TP = PairTile<T>::QP;
   if (items >= kTiledMin && std::max(ph, pl) <= 10)
     // Both positions <= 10: 1024 x 1 blocks (30q complex128: 6.07-6.23 TB/s against 6.02-6.08
     // with the small blocks, which are faster above).

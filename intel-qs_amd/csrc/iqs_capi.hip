@@ -165,12 +165,17 @@ constexpr int kTB = 1024, kTP = 2;  // pair and 4x4 kernels for complex64 (tiled
 // QB x QP: the dense 4x4 kernel with a position above 10 (256 x 1 for both precisions:
 // 30q complex128 at (5,20), (8,25), (12,13), (20,21), (24,25), (28,29), (15,29) averaged
 // 6.14 -> 6.27 TB/s against 128 x 2, 32q complex64 6.16 -> 6.30 against 1024 x 1).
-// CB x CP: pairs with two inserted bits (controlled 2x2, swap family) from the line width up
-// and above position 10: complex64 256 x 1 (32q: 12 controlled and swap cases averaged
-// 6.07 -> 6.27 TB/s against 1024 x 2, every case faster); complex128 keeps 128 x 2 (256 x 1
-// was faster on swaps and slower on controlled gates, 6.27 -> 6.32 on average).
-template <typename T> struct PairTile { static constexpr int B = 1024, P = 2, QB = 256, QP = 1, CB = 256, CP = 1; };
-template <> struct PairTile<double> { static constexpr int B = 128, P = 2, QB = 256, QP = 1, CB = 128, CP = 2; };
+// CB x CP: pairs with two or more inserted bits (controlled 2x2, swap family) from the line
+// width up and above position 10: complex64 256 x 1 (32q: 12 controlled and swap cases
+// averaged 6.07 -> 6.27 TB/s against 1024 x 2, every case faster); complex128 256 x 1 for
+// moves (SWAP, CNOT: 30q swaps 6.15 -> 6.39 against 128 x 2) and 128 x 2 for products
+// (256 x 1 ran controlled SU(2) gates 2-4% slower).
+template <typename T> struct PairTile {
+  static constexpr int B = 1024, P = 2, QB = 256, QP = 1, CB = 256, CP = 1, MB = 256, MP = 1;
+};
+template <> struct PairTile<double> {
+  static constexpr int B = 128, P = 2, QB = 256, QP = 1, CB = 128, CP = 2, MB = 256, MP = 1;
+};
 constexpr int kLB = 256, kLP = 2;   // lane-shuffle 1q kernel
 // Lane-shuffle 1q kernel and 2q diagonal, per precision (30q sweeps): complex128 lane kernel
 // 64 x 2 (one wavefront per workgroup) 6.36 -> 6.60 TB/s against 256 x 2, complex64 keeps
@@ -208,10 +213,11 @@ int launch_pair_m(iqsh_handle h, const Ins<K>& ins, uint64_t mask0, uint64_t mas
     // dense 1q targets 6-10: 6.44 TB/s at 2^11, 6.57 at 2^10).
     k_pair<T, K, MOVE, true, TB, TP, kSub, 10><<<grid, TB, 0, g_stream>>>((V*)h->d, items, ins, mask0, mask1,
                                                                             mat2<T>(m));
-  else if (items >= kTiledMin && K >= 2)
-    k_pair<T, K, MOVE, true, PairTile<T>::CB, PairTile<T>::CP><<<grid2(items / (PairTile<T>::CB * PairTile<T>::CP)),
-                                                                 PairTile<T>::CB, 0, g_stream>>>((V*)h->d, items, ins,
-                                                                                                  mask0, mask1, mat2<T>(m));
+  else if (items >= kTiledMin && K >= 2) {
+    constexpr int GB = MOVE ? PairTile<T>::MB : PairTile<T>::CB, GP = MOVE ? PairTile<T>::MP : PairTile<T>::CP;
+    k_pair<T, K, MOVE, true, GB, GP><<<grid2(items / (GB * GP)), GB, 0, g_stream>>>((V*)h->d, items, ins, mask0, mask1,
+                                                                                    mat2<T>(m));
+  }
   else if (items >= kTiledMin)
     k_pair<T, K, MOVE, true, TB, TP><<<grid, TB, 0, g_stream>>>((V*)h->d, items, ins, mask0, mask1, mat2<T>(m));
   else

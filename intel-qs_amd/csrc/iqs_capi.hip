@@ -170,11 +170,14 @@ constexpr int kTB = 1024, kTP = 2;  // pair and 4x4 kernels for complex64 (tiled
 // averaged 6.07 -> 6.27 TB/s against 1024 x 2, every case faster); complex128 256 x 1 for
 // moves (SWAP, CNOT: 30q swaps 6.15 -> 6.39 against 128 x 2) and 128 x 2 for products
 // (256 x 1 ran controlled SU(2) gates 2-4% slower).
+// SB x SP: dense 1q with the target from position 6 up, outside the complex128 <= 10 rule
+// (complex64 256 x 1: 32q targets 6-31 averaged 6.43 -> 6.63 TB/s against 1024 x 2, every
+// target faster; complex128 128 x 2 is the headline kernel, at the in-place scale rate).
 template <typename T> struct PairTile {
-  static constexpr int B = 1024, P = 2, QB = 256, QP = 1, CB = 256, CP = 1, MB = 256, MP = 1;
+  static constexpr int B = 1024, P = 2, QB = 256, QP = 1, CB = 256, CP = 1, MB = 256, MP = 1, SB = 256, SP = 1;
 };
 template <> struct PairTile<double> {
-  static constexpr int B = 128, P = 2, QB = 256, QP = 1, CB = 128, CP = 2, MB = 256, MP = 1;
+  static constexpr int B = 128, P = 2, QB = 256, QP = 1, CB = 128, CP = 2, MB = 256, MP = 1, SB = 128, SP = 2;
 };
 constexpr int kLB = 256, kLP = 2;   // lane-shuffle 1q kernel
 // Lane-shuffle 1q kernel and 2q diagonal, per precision (30q sweeps): complex128 lane kernel
@@ -218,9 +221,12 @@ int launch_pair_m(iqsh_handle h, const Ins<K>& ins, uint64_t mask0, uint64_t mas
     k_pair<T, K, MOVE, true, GB, GP><<<grid2(items / (GB * GP)), GB, 0, g_stream>>>((V*)h->d, items, ins, mask0, mask1,
                                                                                     mat2<T>(m));
   }
-  else if (items >= kTiledMin)
-    k_pair<T, K, MOVE, true, TB, TP><<<grid, TB, 0, g_stream>>>((V*)h->d, items, ins, mask0, mask1, mat2<T>(m));
-  else
+  else if (items >= kTiledMin) {
+    // one inserted bit here (two or more took the branch above)
+    constexpr int GB = PairTile<T>::SB, GP = PairTile<T>::SP;
+    k_pair<T, K, MOVE, true, GB, GP><<<grid2(items / (GB * GP)), GB, 0, g_stream>>>((V*)h->d, items, ins, mask0, mask1,
+                                                                                    mat2<T>(m));
+  } else
     k_pair<T, K, MOVE, false, kSB, kSP><<<grid2(grid_for(items, kSB * kSP)), kSB, 0, g_stream>>>((V*)h->d, items, ins, mask0,
                                                                                           mask1, mat2<T>(m));
   CHECK_LAUNCH();

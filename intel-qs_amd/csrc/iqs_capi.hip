@@ -191,7 +191,9 @@ template <typename T> struct DiagTile { static constexpr int B = 128, P = 2; };
 // Lane-shuffle 4x4, for a higher position >= 6 and below it: complex64 64 x 4 everywhere
 // (30q: (1,0) 4.98 -> 5.63 TB/s, (29,0) 5.86 -> 6.14); complex128 64 x 2 above ((0,29)
 // 5.83 -> 6.08, swap (1,28) 2.74 -> 3.02 algorithmic), 256 x 2 below (5.45 against 5.15).
-template <typename T> struct QuadLaneTile { static constexpr int B = 64, P = 4, BL = 64, PL = 4; };
+// B x P: the lane 4x4 with its higher position >= 6; complex64 256 x 1 (32q: dense (0,31),
+// (3,29) and swaps (0,31), (1,30), (2,29) 1-8% faster than 64 x 4, (5,20) equal).
+template <typename T> struct QuadLaneTile { static constexpr int B = 256, P = 1, BL = 64, PL = 4; };
 template <> struct QuadLaneTile<double> { static constexpr int B = 64, P = 2, BL = kLB, PL = kLP; };
 
 // Index bits inside one 128-byte cache line: 3 for complex128, 4 for complex64.

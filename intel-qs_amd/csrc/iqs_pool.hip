@@ -133,18 +133,18 @@ __device__ __forceinline__ uint64_t pool_tile() {
   return (G & 7) ? b : xcd_block(b, G);
 }
 
-template <typename T, bool NT>
-__global__ __launch_bounds__(kTB) void k_pool_1q_t(typename Vec<T>::t* __restrict__ st, unsigned pos, unsigned sbits,
-                                                   const Mat2<T>* __restrict__ mats, uint64_t nstates)
+template <typename T, bool NT, int TB = kTB, int TP = kTP>
+__global__ __launch_bounds__(TB) void k_pool_1q_t(typename Vec<T>::t* __restrict__ st, unsigned pos, unsigned sbits,
+                                                  const Mat2<T>* __restrict__ mats, uint64_t nstates)
 {
   using V = typename Vec<T>::t;
-  const uint64_t hb = uint64_t(1) << pos, k0 = pool_tile() * (kTB * kTP) + threadIdx.x;
-  V a[kTP], b[kTP];
-  uint64_t i0[kTP];
-  bool ok[kTP];
+  const uint64_t hb = uint64_t(1) << pos, k0 = pool_tile() * (TB * TP) + threadIdx.x;
+  V a[TP], b[TP];
+  uint64_t i0[TP];
+  bool ok[TP];
 #pragma unroll
-  for (int j = 0; j < kTP; ++j) {
-    i0[j] = ins0(k0 + (uint64_t)j * kTB, pos);
+  for (int j = 0; j < TP; ++j) {
+    i0[j] = ins0(k0 + (uint64_t)j * TB, pos);
     ok[j] = (i0[j] >> sbits) < nstates;
     if (ok[j]) {
       a[j] = ldx<NT>(&st[i0[j]]);
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kTB) void k_pool_1q_t(typename Vec<T>::t* __restric
     }
   }
 #pragma unroll
-  for (int j = 0; j < kTP; ++j) {
+  for (int j = 0; j < TP; ++j) {
     if (!ok[j]) continue;
     const Mat2<T> m = mats[i0[j] >> sbits];
     stx<NT>(cmad2(m.m00, a[j], m.m01, b[j]), &st[i0[j]]);
@@ -410,7 +410,12 @@ int do_1q_each(iqsh_handle h, unsigned sbits, unsigned pos, const double* mats, 
   } else if (items % (kTB * kTP) == 0) {
     // items is a power of two: a whole number of tiles, 8 or more from 2^12 amplitudes up
     const dim3 g = tile_grid(items / (kTB * kTP));
-    if (pos >= pool_line_bits(sizeof(V)))
+    // complex64 from the line width up: 256 x 1 tiles (8 x 28 qubits: position 5 6.25 ->
+    // 6.47 TB/s, 13 6.43 -> 6.54, top equal)
+    if (pos >= pool_line_bits(sizeof(V)) && sizeof(T) == 4)
+      k_pool_1q_t<T, true, 2 * kTB, 1><<<tile_grid(items / (2 * kTB)), 2 * kTB, 0, iqsi::stream()>>>(
+          (V*)h->d, pos, sbits, (const Mat2<T>*)g_pw.mats, nstates);
+    else if (pos >= pool_line_bits(sizeof(V)))
       k_pool_1q_t<T, true><<<g, kTB, 0, iqsi::stream()>>>((V*)h->d, pos, sbits, (const Mat2<T>*)g_pw.mats, nstates);
     else
       k_pool_1q_t<T, false><<<g, kTB, 0, iqsi::stream()>>>((V*)h->d, pos, sbits, (const Mat2<T>*)g_pw.mats, nstates);

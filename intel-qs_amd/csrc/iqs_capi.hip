@@ -218,6 +218,11 @@ int launch_pair_m(iqsh_handle h, const Ins<K>& ins, uint64_t mask0, uint64_t mas
     // dense 1q targets 6-10: 6.44 TB/s at 2^11, 6.57 at 2^10).
     k_pair<T, K, MOVE, true, TB, TP, kSub, 10><<<grid, TB, 0, g_stream>>>((V*)h->d, items, ins, mask0, mask1,
                                                                             mat2<T>(m));
+  else if (items >= kTiledMin && K >= 2 && sizeof(T) == 8 && ins.p[0] < 6)
+    // Complex128 with the lowest position 3-5 (inside a wavefront, from the line width up):
+    // 128 x 1 (30q: controlled (3,20) 5.42 -> 6.16 TB/s, swap (4,25) 5.82 -> 6.57).
+    k_pair<T, K, MOVE, true, 128, 1><<<grid2(items / 128), 128, 0, g_stream>>>((V*)h->d, items, ins, mask0, mask1,
+                                                                               mat2<T>(m));
   else if (items >= kTiledMin && K >= 2) {
     constexpr int GB = MOVE ? PairTile<T>::MB : PairTile<T>::CB, GP = MOVE ? PairTile<T>::MP : PairTile<T>::CP;
     k_pair<T, K, MOVE, true, GB, GP><<<grid2(items / (GB * GP)), GB, 0, g_stream>>>((V*)h->d, items, ins, mask0, mask1,
